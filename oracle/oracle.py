@@ -1,0 +1,124 @@
+"""ctypes binding of oracle/liboracle.so (the C restatement in qk_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the thing measured
+or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build(force: bool = False) -> Path:
+    src = HERE / "qk_oracle.c"
+    if force or not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+        subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", str(LIB),
+                        str(src), "-lm"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(str(LIB))
+        P = C.c_void_p
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.qko_create.restype = P
+        L.qko_create.argtypes = [C.c_int] * 4 + [dp] + [ip] * 10 + [dp] * 3
+        L.qko_destroy.argtypes = [P]
+        L.qko_set_boys_exact.argtypes = [P, C.c_int]
+        L.qko_fmt.argtypes = [C.c_int, C.c_double, dp]
+        L.qko_fmt_exact.argtypes = [C.c_int, C.c_double, dp]
+        L.qko_schwarz.argtypes = [P, dp, dp]
+        L.qko_set_cutoffs.argtypes = [P, dp, dp]
+        L.qko_density_cutoff.argtypes = [P, dp, dp]
+        L.qko_fock.restype = C.c_long
+        L.qko_fock.argtypes = [P, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                               C.POINTER(C.c_long)]
+        L.qko_eri.restype = C.c_double
+        L.qko_eri.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+        if hasattr(L, "qko_grad"):
+            L.qko_grad.restype = C.c_long
+            L.qko_grad.argtypes = [P, dp, dp, C.c_double, C.c_double, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _f(a, dt):
+    """Fortran-ordered array flattened to a C-contiguous buffer."""
+    return np.ascontiguousarray(np.asarray(a, dt).ravel(order="F"))
+
+
+def fmt(M: int, T: float, exact: bool = False) -> np.ndarray:
+    out = np.zeros(M + 1)
+    (lib().qko_fmt_exact if exact else lib().qko_fmt)(M, float(T), out)
+    return out
+
+
+class Oracle:
+    """Holds one system's shell tables (any object with the Basis attributes)."""
+
+    def __init__(self, b):
+        L = lib()
+        self.nbasis, self.nshell, self.jbasis = int(b.nbasis), int(b.nshell), int(b.jbasis)
+        self.natom = int(np.asarray(b.xyz).shape[1])
+        self._keep = [_f(b.xyz, np.float64)] + [_f(getattr(b, n), np.int32) for n in
+                      ("kstart", "kprim", "katom", "ktype", "Qstart", "Qfinal", "Qsbasis",
+                       "Qfbasis", "ksumtype", "KLMN")] + [_f(getattr(b, n), np.float64) for n in
+                                                          ("cons", "gccoeff", "gcexpo")]
+        self.h = L.qko_create(self.natom, self.nshell, self.nbasis, self.jbasis, *self._keep)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().qko_destroy(self.h)
+            self.h = None
+
+    def boys_exact(self, on: bool):
+        lib().qko_set_boys_exact(self.h, int(on))
+
+    def schwarz(self):
+        Y = np.zeros(self.nshell * self.nshell)
+        cp = np.zeros(self.jbasis * self.jbasis)
+        lib().qko_schwarz(self.h, Y, cp)
+        return Y.reshape(self.nshell, self.nshell, order="F"), cp.reshape(self.jbasis, self.jbasis, order="F")
+
+    def set_cutoffs(self, Y, cutprim):
+        lib().qko_set_cutoffs(self.h, _f(Y, np.float64), _f(cutprim, np.float64))
+
+    def fock(self, D, tau=1e-11, prim_limit=1e-20, pair_cut=1e-10, hybrid=1.0, mode="gpu"):
+        """Two-electron operator G(D).  mode 'cpu' follows hfoperator's get2e
+        enumeration; 'gpu' the reference GPU pair list (pair_cut on Ycutoff) with
+        the GPU's Boys order; 'gpu_cpuboys' the same list with the CPU's Boys order."""
+        G = np.zeros(self.nbasis * self.nbasis)
+        npq = C.c_long(0)
+        nq = lib().qko_fock(self.h, _f(D, np.float64), G, tau, prim_limit, pair_cut, hybrid,
+                            {"cpu": 0, "gpu": 1, "gpu_cpuboys": 2}[mode], C.byref(npq))
+        return G.reshape(self.nbasis, self.nbasis, order="F"), int(nq), int(npq.value)
+
+    def eri(self, i, j, k, l):
+        return lib().qko_eri(self.h, i, j, k, l)
+
+    def density_cutoff(self, D):
+        cm = np.zeros(self.nshell * self.nshell)
+        lib().qko_density_cutoff(self.h, _f(D, np.float64), cm)
+        return cm.reshape(self.nshell, self.nshell, order="F")
+
+
+class Fixture:
+    """A golden .npz as an object with the Basis attribute names."""
+
+    def __init__(self, path):
+        z = np.load(path)
+        self.__dict__.update({k: z[k] for k in z.files})
+        self.nbasis, self.nshell, self.jbasis = int(self.nbasis), int(self.nshell), int(self.jbasis)

@@ -1,0 +1,114 @@
+"""Write the golden fixtures from the reference CPU path (container-only).
+
+Runs ``oracle/_ref/qk_ref_driver`` (the reference Fortran built from its own
+sources by ``oracle/ref_build.mk``) on small seeded systems and stores, per
+system, one compressed ``.npz`` beside this script:
+
+  * the reference's shell tables (kstart, kprim, katom, ktype, Qstart, Qfinal,
+    Qsbasis, Qfbasis, ksumtype, KLMN, cons, gccoeff, gcexpo) and xyz in bohr;
+  * Schwarz bounds ``Ycutoff`` (nshell^2) and primitive bounds ``cutprim``
+    (jbasis^2) from schwarzoff (schwarz.f90:12-42);
+  * a seeded synthetic density ``D`` and the two-electron operator ``G = G(D)``
+    returned by hfoperator with a zero one-electron part (hfoperator.f90:24-138);
+  * optionally the four-centre gradient term ``grad4c`` (gradient.f90:261-320).
+
+usage (repository root):  python tests/golden/make_golden.py [system ...]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+from quick_amd.geometry import small_system, to_quick_input  # noqa: E402
+
+DRIVER = ROOT / "oracle" / "_ref" / "qk_ref_driver"
+BASIS_DIR = "/root/reference/basis"
+KEYWORDS = "HF BASIS=6-31GSS CUTOFF=1.0e-11 DENSERMS=1.0e-7"
+
+# system -> (do_fock, do_grad)
+SYSTEMS = {
+    "h2o": (True, True),
+    "ch4": (True, True),
+    "water2": (True, True),
+    "ethane": (True, True),
+    "water4": (True, True),
+}
+
+
+def synthetic_density(xyz_bohr, katom, ksumtype, nbasis, seed):
+    """Symmetric density with entries decaying as exp(-0.25 R^2) between centres."""
+    rng = np.random.default_rng(seed)
+    centre = np.zeros(nbasis, np.int64)
+    for s in range(len(katom)):
+        centre[ksumtype[s] - 1:ksumtype[s + 1] - 1] = katom[s] - 1
+    r = xyz_bohr[:, centre].T
+    R2 = ((r[:, None, :] - r[None, :, :]) ** 2).sum(-1)
+    u = rng.uniform(-1.0, 1.0, (nbasis, nbasis))
+    D = 0.6 * np.exp(-0.25 * R2) * (u + u.T) / 2.0
+    D[np.diag_indices(nbasis)] += 0.5 + 0.5 * rng.random(nbasis)
+    return D
+
+
+def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
+    syms, xyz = small_system(name)
+    env = dict(os.environ, QUICK_BASIS=BASIS_DIR)
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        out = {}
+        for mode in ["setup"] + (["fock"] if fock else []) + (["grad"] if grad else []):
+            kw = KEYWORDS + (" OPT" if mode == "grad" else "")
+            (td / "job.in").write_text(to_quick_input(syms, xyz, kw))
+            od = td / mode
+            od.mkdir()
+            if mode != "setup":
+                out["D"].ravel(order="F").tofile(td / "D.bin")
+            subprocess.run(f"ulimit -s unlimited && {DRIVER} job.in {mode} D.bin {od}",
+                           shell=True, check=True, cwd=td, env=env,
+                           stdout=subprocess.DEVNULL)
+            if mode == "setup":
+                natom, nbasis, nshell, _, jbasis, maxprim = np.fromfile(od / "sizes.bin", np.int32)
+                rd = lambda n, dt=np.float64: np.fromfile(od / f"{n}.bin", dt)
+                out.update(
+                    natom=natom, nbasis=nbasis, nshell=nshell, jbasis=jbasis, maxprim=maxprim,
+                    symbols=np.array(syms), xyz=rd("xyz").reshape(3, natom, order="F"),
+                    kstart=rd("kstart", np.int32), kprim=rd("kprim", np.int32),
+                    katom=rd("katom", np.int32), ktype=rd("ktype", np.int32),
+                    Qstart=rd("Qstart", np.int32), Qfinal=rd("Qfinal", np.int32),
+                    Qsbasis=rd("Qsbasis", np.int32).reshape(nshell, 4, order="F"),
+                    Qfbasis=rd("Qfbasis", np.int32).reshape(nshell, 4, order="F"),
+                    ksumtype=rd("ksumtype", np.int32),
+                    KLMN=rd("KLMN", np.int32).reshape(3, nbasis, order="F"),
+                    cons=rd("cons"),
+                    gccoeff=rd("gccoeff").reshape(maxprim, nbasis, order="F"),
+                    gcexpo=rd("gcexpo").reshape(maxprim, nbasis, order="F"),
+                    Ycutoff=rd("Ycutoff").reshape(nshell, nshell, order="F"),
+                    cutprim=rd("cutprim").reshape(jbasis, jbasis, order="F"),
+                )
+                out["D"] = synthetic_density(out["xyz"], out["katom"], out["ksumtype"],
+                                             int(nbasis), seed)
+            elif mode == "fock":
+                out["G"] = np.fromfile(od / "G.bin").reshape(out["nbasis"], out["nbasis"], order="F")
+                out["time_fock"] = np.fromfile(od / "time_fock.bin")[0]
+            elif mode == "grad":
+                out["grad4c"] = np.fromfile(od / "grad4c.bin").reshape(3, out["natom"], order="F")
+                out["time_grad"] = np.fromfile(od / "time_grad.bin")[0]
+    dst = Path(__file__).with_name(f"{name}.npz")
+    np.savez_compressed(dst, **out)
+    return dst
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(SYSTEMS)
+    for n in names:
+        f, g = SYSTEMS[n]
+        p = run(n, f, g)
+        z = np.load(p)
+        print(p.name, "nbasis", int(z["nbasis"]), "fock s", z.get("time_fock"),
+              "grad s", z.get("time_grad"), f"{p.stat().st_size/1024:.0f} KiB")
