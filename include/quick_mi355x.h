@@ -1,0 +1,166 @@
+/*
+ * quick_mi355x.h -- C-ABI of the MI355X (gfx950) Fock-build library.
+ *
+ * Two layers:
+ *
+ *  1. The reference's Fortran -> C boundary, symbol for symbol (QUICK
+ *     src/cuda/gpu.h:17-50 and the definitions in src/cuda/gpu.cu cited below).
+ *     Every argument is passed by reference, as gfortran/flang pass implicit-
+ *     interface arguments; arrays are Fortran column-major with 1-based index
+ *     contents.  These calls are synchronous, keep one process-global context,
+ *     and never return an error code (as the reference); a failure is recorded
+ *     and readable through qk_last_error(), and fatal ones abort with a message.
+ *
+ *  2. An explicit handle-based API (qk_*) for callers that keep the density and
+ *     the operator resident in device memory (the test harness, bench.py and
+ *     any host that binds through ctypes/iso_c_binding).  These return 0 on
+ *     success and a negative code on failure (message in qk_last_error()).
+ */
+#ifndef QUICK_MI355X_H
+#define QUICK_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- reference boundary (Fortran, by reference) ---------------- */
+
+/* gpu.cu:29   create the process-global context */
+void gpu_startup_(void);
+/* gpu.cu:18   choose the device (-1: first visible device) */
+void gpu_set_device_(int* gpu_dev_id);
+/* gpu.cu:43   initialise the chosen device; aborts if none is present */
+void gpu_init_(void);
+/* gpu.cu:175  device information for the output header (name is written
+ * without a terminator, namelen receives its length) */
+void gpu_get_device_info_(int* gpu_dev_count, int* gpu_dev_id, int* gpu_dev_mem,
+                          int* gpu_num_proc, double* gpu_core_freq, char* gpu_dev_name,
+                          int* name_len, int* majorv, int* minorv);
+/* gpu.cu:221  system sizes */
+void gpu_setup_(int* natom, int* nbasis, int* nElec, int* imult, int* molchg, int* iAtomType);
+/* gpu.cu:283  coordinates, 3 x natom, bohr */
+void gpu_upload_xyz_(double* atom_xyz);
+/* gpu.cu:334  atom types and charges (kept for the XC step; unused by the ERIs) */
+void gpu_upload_atom_and_chg_(int* atom_type, double* atom_chg);
+/* gpu.cu:1070 basis: QUICK's shell tables (quick_basis_module) */
+void gpu_upload_basis_(int* nshell, int* nprim, int* jshell, int* jbasis, int* maxcontract,
+                       int* ncontract, int* itype, double* aexp, double* dcoeff,
+                       int* first_basis_function, int* last_basis_function,
+                       int* first_shell_basis_function, int* last_shell_basis_function,
+                       int* ncenter, int* kstart, int* katom, int* ktype, int* kprim, int* kshell,
+                       int* Ksumtype, int* Qnumber, int* Qstart, int* Qfinal, int* Qsbasis,
+                       int* Qfbasis, double* gccoeff, double* cons, double* gcexpo, int* KLMN);
+/* gpu.cu:401  Schwarz bounds Ycutoff(nshell,nshell) and cutPrim(jbasis,jbasis);
+ * builds the class-sorted, bound-sorted pair lists */
+void gpu_upload_cutoff_matrix_(double* YCutoff, double* cutPrim);
+/* gpu.cu:269  0: HF (K weight 1), 1: B3LYP (0.2), 2: pure DFT (0) */
+void gpu_upload_method_(int* quick_method);
+/* gpu.cu:1008 o: operator to accumulate into (H on entry), dense: density;
+ * co and vec are accepted and unused, as in the reference */
+void gpu_upload_calculated_(double* o, double* co, double* vec, double* dense);
+/* gpu.cu:356  cutMatrix(nshell,nshell) = max |D| over each shell block, used by
+ * the density screen of the next gpu_get2e_; the fourth argument is never read
+ * (three-argument callers exist: hfoperator.f90:81, gradient.f90:283) */
+void gpu_upload_cutoff_(double* cutMatrix, double* integralCutoff, double* primLimit,
+                        double* DMCutoff);
+/* gpu.cu:1806 forward: o <- o + G(D), symmetric, written in place */
+void gpu_get2e_(double* o);
+/* gpu.cu:1393 gradient seed (3*natom) and gradient cutoff */
+void gpu_upload_grad_(double* grad, double* gradCutoff);
+/* gpu.cu:1481 backward: grad <- grad + dE_2e/dR for the uploaded density */
+void gpu_grad_(double* grad);
+/* gpu.cu:1866 XC on the GPU: not part of this library; records an error */
+void gpu_getxc_(int* isg, double* sigrad2, double* Eelxc, double* aelec, double* belec,
+                double* o);
+/* gpu.cu:1991 / gpu.cu:1559 NODIRECT (stored-integral) mode: not part of this
+ * library; records an error */
+void gpu_aoint_(double* leastIntegralCutoff, double* maxIntegralCutoff, int* intNum,
+                char* intFileName);
+void gpu_addint_(double* o, int* intindex, char* intFileName);
+/* gpu.cu:1445 free per-geometry state; gpu.cu:206 free everything */
+void gpu_cleanup_(void);
+void gpu_shutdown_(void);
+
+/* ---------------- explicit API ---------------- */
+
+typedef struct qk_ctx qk_ctx;
+
+enum qk_precision { QK_FP64 = 0, QK_MIXED = 1 };
+enum qk_boys_order { QK_BOYS_ORDER_SHELL = 0, QK_BOYS_ORDER_QSHELL = 1 };
+
+typedef struct qk_opts {
+    double tau;            /* quartet cutoff (QUICK integralCutoff), e.g. 1e-11 */
+    double prim_limit;     /* primitive cutoff (QUICK primLimit), e.g. 1e-20 */
+    double pair_cut;       /* drop shell pairs with Ycutoff <= pair_cut (0: keep all,
+                              as the reference CPU path; 1e-10: the reference GPU) */
+    double hybrid;         /* exchange weight: 1 HF, 0.2 B3LYP, 0 pure DFT */
+    int boys_order;        /* which order QUICK's FmT is asked for (branch selection
+                              near T = 0): QK_BOYS_ORDER_SHELL = CPU path
+                              (shell.f90:509), QK_BOYS_ORDER_QSHELL = GPU path */
+    int precision;         /* QK_FP64 or QK_MIXED */
+    double fp32_threshold; /* QK_MIXED: quartets with Y_ij*Y_kl below it run in FP32 */
+} qk_opts;
+
+/* fills the defaults: tau 1e-11, prim_limit 1e-20, pair_cut 0, hybrid 1,
+ * shell Boys order, FP64, fp32_threshold 1e-5 */
+void qk_default_opts(qk_opts* o);
+
+const char* qk_last_error(void);
+qk_ctx* qk_create(int device);
+void qk_destroy(qk_ctx* c);
+/* the process-global context the gpu_*_ entry points use (NULL before gpu_startup_) */
+qk_ctx* qk_global_context(void);
+
+/* Same arrays as gpu_upload_xyz_/gpu_upload_basis_ (column-major, 1-based contents). */
+int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell, int nbasis, int jbasis,
+                  const int* kstart, const int* kprim, const int* katom, const int* ktype,
+                  const int* Qstart, const int* Qfinal, const int* Qsbasis, const int* Qfbasis,
+                  const int* ksumtype, const int* KLMN, const double* cons,
+                  const double* gccoeff, const double* gcexpo);
+/* Schwarz bounds from the caller (as gpu_upload_cutoff_matrix_) */
+int qk_set_schwarz(qk_ctx* c, const double* Ycutoff, const double* cutprim);
+/* Schwarz bounds computed on the device with QUICK's definitions
+ * (schwarz.f90:12-548); outputs (host, may be NULL) receive them too */
+int qk_compute_schwarz(qk_ctx* c, double* Ycutoff_out, double* cutprim_out);
+/* Build the class/bound-sorted work lists for the given options (tau, pair_cut,
+ * precision split).  Called implicitly by the builds when the options change. */
+int qk_prepare(qk_ctx* c, const qk_opts* o);
+
+/* Forward: dG <- G(dD), the two-electron operator (full symmetric n x n) for
+ * the density dD; both device pointers, n x n doubles.  Runs on `stream`
+ * (a hipStream_t, NULL = default) and returns after enqueueing. */
+int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream);
+/* Host convenience: F <- F + G(D) (host arrays), synchronous */
+int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
+
+/* Backward: dgrad (3*natom, device) <- dgrad + dE_2e/dR for density dD. */
+int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
+int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
+
+/* Statistics of the last build: surviving contracted Qshell quartets, primitive
+ * quartets evaluated (FP64 and FP32), algorithmic flops. */
+typedef struct qk_stats {
+    long long quartets;
+    long long quartets_fp32;
+    long long prim_quartets;
+    double flops_fp64;
+    double flops_fp32;
+} qk_stats;
+int qk_last_stats(qk_ctx* c, qk_stats* s);
+
+/* Per-class launch bookkeeping for profiling: number of classes, and the
+ * (la,lb,lc,ld, quartet-space size) of class k. */
+int qk_num_classes(void);
+int qk_class_info(qk_ctx* c, int k, int* lalblcld, long long* work);
+/* Record HIP events around every class kernel of the following builds (on the
+ * stream the kernels run on). */
+int qk_set_timing(qk_ctx* c, int on);
+/* Figures of class k in the last build: surviving quartets, primitive
+ * quartets, algorithmic flops; and the class kernel's average time in ms over
+ * all builds since qk_set_timing(c, 1) (0 unless timing was on). */
+int qk_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -6,15 +6,19 @@
 ! density read from a file:
 !   mode "fock": hfoperator with a zero one-electron operator
 !                (hfoperator.f90:24-138), i.e. the two-electron part G(D);
-!   mode "grad": the four-centre gradient loop of hfgrad (gradient.f90:261-320).
+!   mode "grad": the four-centre gradient loop of hfgrad (gradient.f90:261-320);
+!   mode "fock_sample <stride>": the CPU Fock build's outer loop (hfoperator.f90:
+!                109-111) restricted to every stride-th shell II, timed -- a
+!                bounded sample for the CPU baseline.
 ! Everything is dumped as raw little-endian binary so the Python fixture writer
 ! can read it without running any reference code itself.
 !
-! usage: qk_ref_driver <job.in> <mode> <density.bin> <outdir>
+! usage: qk_ref_driver <job.in> <mode> <density.bin> <outdir> [stride]
 program qk_ref_driver
    use allmod
    implicit none
-   integer :: ierr, i, j, nb, ios
+   integer :: ierr, i, j, nb, ios, stride, nsamp
+   character(len=32) :: sarg
    character(len=256) :: mode, dfile, odir
    double precision, allocatable :: oneElecO(:,:)
    double precision :: Testtmp, testCutoff, cutoffTest, DNtemp, t0, t1
@@ -70,6 +74,19 @@ program qk_ref_driver
       call cpu_time(t1)
       call dump_d(trim(odir)//'/G.bin', quick_qm_struct%o, nb*nb)
       call dump_d(trim(odir)//'/time_fock.bin', (/ t1-t0 /), 1)
+   else if (trim(mode) == 'fock_sample') then
+      call getarg(5, sarg)
+      read(sarg, *) stride
+      quick_qm_struct%o = 0.0d0
+      call densityCutoff()
+      nsamp = 0
+      call cpu_time(t0)
+      do II = max(1, stride / 2), jshell, stride
+         call get2e(II)
+         nsamp = nsamp + 1
+      enddo
+      call cpu_time(t1)
+      call dump_d(trim(odir)//'/time_sample.bin', (/ t1-t0, dble(nsamp), dble(jshell) /), 3)
    else if (trim(mode) == 'grad') then
       quick_qm_struct%gradient = 0.0d0
       call cpu_time(t0)

@@ -212,3 +212,19 @@ def qshells(b: Basis):
             f0 = b.ksumtype[s] - 1 + b.Qsbasis[s, L]
             out.append((s, L, f0, nf[L]))
     return out
+
+
+def write_quick_basis_file(path, data: dict | None = None) -> None:
+    """Write basis data in the text layout QUICK's readbasis parses
+    (basis.f90:386-400): an "<element>  0" line, then per shell "<type> <nprim>
+    1.00" followed by one line per primitive, and "****" after each element."""
+    data = data or load_basis_data()
+    lines = []
+    for el, shells in data.items():
+        lines.append(f"{el:<6s}0")
+        for sh in shells:
+            lines.append(f"{sh['type']:<6s}{len(sh['prims'])}     1.00")
+            for pr in sh["prims"]:
+                lines.append("    " + "".join(f"{float(v)!r:>24}" for v in pr))
+        lines.append("****")
+    Path(path).write_text("\n".join(lines) + "\n")

@@ -1,0 +1,324 @@
+"""Generates the per-class ERI device code (VRR + HRR) for the HIP kernels.
+
+For every quartet class (la lb|lc ld) with la >= lb, lc >= ld and bra pair class
+>= ket pair class (s,p,d: 21 classes) this emits straight-line code for
+
+  * ``vrr_<tag>``: the Obara-Saika / Head-Gordon-Pople vertical recurrence
+    [e0|f0]^(m) from F_m(T) (already multiplied by the primitive prefactor), for
+    |e| in [la, la+lb] and |f| in [lc, lc+ld], accumulated into ``acc`` -- only
+    the intermediates the targets need are emitted, in dependency order;
+  * ``hrr_<tag>``: the horizontal recurrences (a,b+1_i) = (a+1_i,b) + AB_i (a,b)
+    on the bra and (c,d+1_i) = (c+1_i,d) + CD_i (c,d) on the ket, producing
+    (ab|cd) in QUICK's component order (p: x,y,z; d: xx,xy,yy,xz,yz,zz).
+
+The recurrence variables are the ones QUICK's `vertical` takes
+(subs/hrr.f90:1018; gpu_get2e_subs.h:679-682): P-A, W-P, Q-C, W-Q, 1/2zeta,
+1/2eta, 1/2(zeta+eta), eta/(zeta+eta), zeta/(zeta+eta).  The direction chosen
+at every step (first non-zero Cartesian index) is the one the CPU oracle uses,
+so both evaluate the same arithmetic expressions.
+
+usage: python gen_eri.py <out.inc>
+"""
+from __future__ import annotations
+
+import sys
+from functools import lru_cache
+
+LMAX = 8
+
+
+def carts(L):
+    """Cartesian exponent triples of total L in the oracle's index order."""
+    out = []
+    for i in range(L, -1, -1):
+        for j in range(L - i, -1, -1):
+            out.append((i, j, L - i - j))
+    return out
+
+
+# QUICK's component order within a shell (basis.f90:415-574)
+QUICK_ORDER = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+               2: [(2, 0, 0), (1, 1, 0), (0, 2, 0), (1, 0, 1), (0, 1, 1), (0, 0, 2)]}
+
+PAIR_CLASSES = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2)]
+
+
+def quartet_classes():
+    out = []
+    for bi, bra in enumerate(PAIR_CLASSES):
+        for ki, ket in enumerate(PAIR_CLASSES):
+            if ki <= bi:
+                out.append(bra + ket)
+    return out
+
+
+def tag(c):
+    return "".join("spdf"[x] for x in c)
+
+
+def erange(lo, hi):
+    """Cartesians with lo <= |e| <= hi, in order of increasing |e|."""
+    out = []
+    for L in range(lo, hi + 1):
+        out += carts(L)
+    return out
+
+
+def first_dir(v):
+    return 0 if v[0] > 0 else (1 if v[1] > 0 else 2)
+
+
+def sub(v, i, n=1):
+    w = list(v)
+    w[i] -= n
+    return tuple(w)
+
+
+def name_of(e, f, m):
+    return "v_%d%d%d_%d%d%d_%d" % (e + f + (m,))
+
+
+def gen_vrr(cls):
+    la, lb, lc, ld = cls
+    E, F = la + lb, lc + ld
+    targets = [(e, f, 0) for e in erange(la, E) for f in erange(lc, F)]
+    order, seen = [], set()
+
+    def visit(node):
+        if node in seen:
+            return
+        e, f, m = node
+        deps = []
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            deps += [(e, f1, m), (e, f1, m + 1)]
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                deps += [(e, f2, m), (e, f2, m + 1)]
+            if e[i] > 0:
+                deps.append((sub(e, i), f1, m + 1))
+        elif sum(e) > 0:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            deps += [(e1, f, m), (e1, f, m + 1)]
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                deps += [(e2, f, m), (e2, f, m + 1)]
+        for d in deps:
+            visit(d)
+        seen.add(node)
+        order.append(node)
+
+    for t in targets:
+        visit(t)
+
+    X = "xyz"
+    lines = []
+    nflop = 0
+    for node in order:
+        e, f, m = node
+        nm = name_of(e, f, m)
+        if sum(e) == 0 and sum(f) == 0:
+            lines.append(f"    const R {nm} = F[{m}];")
+            continue
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            expr = f"QC{X[i]} * {name_of(e, f1, m)} + WQ{X[i]} * {name_of(e, f1, m + 1)}"
+            nflop += 3
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                expr += f" + R({f1[i]}) * oo2e * ({name_of(e, f2, m)} - roe * {name_of(e, f2, m + 1)})"
+                nflop += 5
+            if e[i] > 0:
+                expr += f" + R({e[i]}) * oo2ze * {name_of(sub(e, i), f1, m + 1)}"
+                nflop += 3
+        else:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            expr = f"PA{X[i]} * {name_of(e1, f, m)} + WP{X[i]} * {name_of(e1, f, m + 1)}"
+            nflop += 3
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                expr += f" + R({e1[i]}) * oo2z * ({name_of(e2, f, m)} - roz * {name_of(e2, f, m + 1)})"
+                nflop += 5
+        lines.append(f"    const R {nm} = {expr};")
+    ne, nf = len(erange(la, E)), len(erange(lc, F))
+    acc = []
+    for a, e in enumerate(erange(la, E)):
+        for b, f in enumerate(erange(lc, F)):
+            acc.append(f"    acc[{a * nf + b}] += {name_of(e, f, 0)};")
+            nflop += 1
+    body = "\n".join(lines + acc)
+    sig = (f"template <typename R>\n__device__ __forceinline__ void vrr_{tag(cls)}(const R* __restrict__ F, "
+           "R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, "
+           "R oo2z, R oo2e, R oo2ze, R roz, R roe, R* __restrict__ acc)")
+    return f"{sig}\n{{\n{body}\n}}\n", ne * nf, nflop, len(order)
+
+
+def gen_hrr(cls):
+    """acc[e][f] -> out[a][b][c][d] in QUICK's component order."""
+    la, lb, lc, ld = cls
+    E, F = la + lb, lc + ld
+    elist, flist = erange(la, E), erange(lc, F)
+    eidx = {e: k for k, e in enumerate(elist)}
+    fidx = {f: k for k, f in enumerate(flist)}
+    nf = len(flist)
+    X = "xyz"
+    lines = []
+    nflop = 0
+
+    # bra HRR for every f: names hb_<a>_<b>_<f>
+    def hb(a, b, fk):
+        return "hb_%d%d%d_%d%d%d_%d" % (a + b + (fk,))
+
+    @lru_cache(maxsize=None)
+    def need_bra(a, b):
+        return True
+
+    bra_terms = {}
+
+    def bra_expr(a, b, fk):
+        key = (a, b, fk)
+        if key in bra_terms:
+            return bra_terms[key]
+        if sum(b) == 0:
+            nm = f"acc[{eidx[a] * nf + fk}]"
+            bra_terms[key] = nm
+            return nm
+        i = first_dir(b)
+        b1 = sub(b, i)
+        a1 = list(a)
+        a1[i] += 1
+        a1 = tuple(a1)
+        x1 = bra_expr(a1, b1, fk)
+        x2 = bra_expr(a, b1, fk)
+        nm = hb(a, b, fk)
+        lines.append(f"    const R {nm} = {x1} + AB{X[i]} * {x2};")
+        nonlocal_flops[0] += 2
+        bra_terms[key] = nm
+        return nm
+
+    nonlocal_flops = [0]
+    A_list, B_list = QUICK_ORDER[la], QUICK_ORDER[lb]
+    C_list, D_list = QUICK_ORDER[lc], QUICK_ORDER[ld]
+    # (ab|f0) for the needed f: all f in flist
+    for a in A_list:
+        for b in B_list:
+            for fk in range(nf):
+                bra_expr(a, b, fk)
+
+    ket_terms = {}
+
+    def ket_expr(abkey, c, d):
+        key = (abkey, c, d)
+        if key in ket_terms:
+            return ket_terms[key]
+        a, b = abkey
+        if sum(d) == 0:
+            nm = bra_terms[(a, b, fidx[c])]
+            ket_terms[key] = nm
+            return nm
+        i = first_dir(d)
+        d1 = sub(d, i)
+        c1 = list(c)
+        c1[i] += 1
+        c1 = tuple(c1)
+        x1 = ket_expr(abkey, c1, d1)
+        x2 = ket_expr(abkey, c, d1)
+        nm = "hk_%d%d%d_%d%d%d_%d%d%d_%d%d%d" % (a + b + c + d)
+        lines.append(f"    const R {nm} = {x1} + CD{X[i]} * {x2};")
+        nonlocal_flops[0] += 2
+        ket_terms[key] = nm
+        return nm
+
+    outs = []
+    k = 0
+    for a in A_list:
+        for b in B_list:
+            for c in C_list:
+                for d in D_list:
+                    outs.append(f"    out[{k}] = {ket_expr((a, b), c, d)};")
+                    k += 1
+    sig = (f"template <typename R>\n__device__ __forceinline__ void hrr_{tag(cls)}(const R* __restrict__ acc, "
+           "R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, R* __restrict__ out)")
+    return f"{sig}\n{{\n" + "\n".join(lines + outs) + "\n}\n", nonlocal_flops[0], k
+
+
+def write_units(outdir):
+    """One translation unit per quartet class (and per Schwarz diagonal class),
+    so hipcc compiles the heavy classes in parallel."""
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    names = []
+    for cls in quartet_classes():
+        t = tag(cls)
+        body = ['// Generated by quick_amd/csrc/gen_eri.py -- do not edit.', '#include "../qk_kernels.h"',
+                'namespace qk {',
+                f'void launch_fock_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
+                f'    hipLaunchKernelGGL((fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), 0, s, a, w);',
+                '}',
+                f'size_t fock_scratch_{t}() {{',
+                '    hipFuncAttributes at{};',
+                f'    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>)) != hipSuccess) return 0;',
+                '    return at.localSizeBytes;',
+                '}']
+        if cls[:2] == cls[2:]:
+            pt = t[:2]
+            body += [f'void launch_schwarz_{pt}(const int* pairs, int np, const int* qa, const int* qb, const int* prim0,',
+                     '                        const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz,',
+                     '                        const double* cons, double* ymax, double* pmax, hipStream_t s) {',
+                     f'    hipLaunchKernelGGL((schwarz_kernel<{cls[0]}, {cls[1]}>), dim3((np + 255) / 256), dim3(256), 0, s,',
+                     '                       pairs, np, qa, qb, prim0, nprim, qs, pp, xyz, cons, ymax, pmax);',
+                     '}']
+        body.append('}  // namespace qk')
+        fn = os.path.join(outdir, f'fock_{t}.hip')
+        new = "\n".join(body) + "\n"
+        if not os.path.exists(fn) or open(fn).read() != new:
+            open(fn, 'w').write(new)
+        names.append(fn)
+    return names
+
+
+def main(path):
+    out = ["// Generated by quick_amd/csrc/gen_eri.py -- do not edit.",
+           "// Per-class VRR/HRR for (la lb|lc ld), la>=lb, lc>=ld, bra pair class >= ket pair class.",
+           "#pragma once", ""]
+    table = []
+    for cls in quartet_classes():
+        v, nacc, vflop, nterms = gen_vrr(cls)
+        h, hflop, nout = gen_hrr(cls)
+        out += [v, h]
+        table.append((cls, nacc, vflop, hflop, nout))
+    hdr = ["// Generated by quick_amd/csrc/gen_eri.py -- do not edit.",
+           "// Arithmetic operations of the generated code, per class (FMA = 2):",
+           "// la, lb, lc, ld, accumulator size, VRR ops per primitive quartet, HRR ops per quartet, outputs",
+           "#pragma once",
+           "static const int kEriFlopTable[][8] = {"]
+    for cls, nacc, vflop, hflop, nout in table:
+        hdr.append(f"    {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, {nacc}, {vflop}, {hflop}, {nout}}},")
+    hdr.append("};")
+    import os
+    fpath = os.path.join(os.path.dirname(path), "eri_flops.h")
+    newh = "\n".join(hdr) + "\n"
+    if not os.path.exists(fpath) or open(fpath).read() != newh:
+        open(fpath, "w").write(newh)
+    out.append("// class, accumulator size, VRR flops per primitive quartet, HRR flops, outputs")
+    out.append("struct EriClassInfo { int la, lb, lc, ld, nacc, vrr_flops, hrr_flops, nout; };")
+    out.append("static const EriClassInfo kEriClassInfo[] = {")
+    for cls, nacc, vflop, hflop, nout in table:
+        out.append(f"    {{{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, {nacc}, {vflop}, {hflop}, {nout}}},")
+    out.append("};")
+    new = "\n".join(out) + "\n"
+    import os
+    if not os.path.exists(path) or open(path).read() != new:
+        open(path, "w").write(new)
+    for cls, nacc, vflop, hflop, nout in table:
+        print(tag(cls), nacc, vflop, hflop, nout)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])
+    if len(sys.argv) > 2:
+        write_units(sys.argv[2])

@@ -1,0 +1,127 @@
+// qk_fock.hip -- Fock-build kernels for MI355X (gfx950).
+//
+// One kernel instantiation per quartet class (la lb|lc ld).  Each thread owns
+// one contracted Qshell quartet at a time: it screens it (Schwarz x density,
+// hfoperator.f90:725-736 / gpu_get2e_subs.h:323-328), runs the primitive loop
+// (Boys F_m(T) with QUICK's fit, VRR accumulated into the contracted [e0|f0]),
+// applies the HRR, and digests the quartet's integrals into the Fock
+// accumulator with the 8-fold-symmetry weights.  The accumulator is int64
+// fixed point (2^-48 units); integer atomics make the result independent of
+// the order quartets finish in, so every build is bitwise reproducible.
+//
+// Work layout: the quartet space of a class is a list of "combos" (bra group x
+// ket group, see qk_internal.h) with per-bra prefix offsets; a flat thread
+// index is decoded with two binary searches.  Consecutive lanes of a wavefront
+// therefore walk consecutive kets of the same bra: the bra's pair and primitive
+// data are broadcast loads and the ket's are coalesced.
+#include "qk_kernels.h"
+
+namespace qk {
+
+// ---------------------------------------------------------------------------
+// helper kernels
+
+// Dp = diag(cons) D diag(cons): folds QUICK's per-function cons factors
+// (applied inside hrrwhole, subs/hrr.f90:261-295) into the density; the
+// operator is scaled back by the same factors in fock_finalize.
+__global__ void scale_density(const double* __restrict__ D, const double* __restrict__ cons, double* __restrict__ Dp,
+                              int n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * n) return;
+    const int i = int(k / n), j = int(k % n);
+    Dp[k] = cons[i] * cons[j] * D[k];
+}
+
+// cutmatrix(II,JJ) = max |D| over the (II,JJ) shell block (schwarz.f90:550-599)
+__global__ void density_cutoff(const double* __restrict__ D, const int* __restrict__ sh_f0,
+                               const int* __restrict__ sh_nf, double* __restrict__ cm, int n, int ns) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)ns * ns) return;
+    const int I = int(k / ns), J = int(k % ns);
+    double m = 0.0;
+    for (int i = sh_f0[I]; i < sh_f0[I] + sh_nf[I]; ++i)
+        for (int j = sh_f0[J]; j < sh_f0[J] + sh_nf[J]; ++j) m = fmax(m, fabs(D[(long long)j * n + i]));
+    cm[k] = m;
+}
+
+__global__ void fill_u64(unsigned long long* p, long long count) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) p[k] = 0ull;
+}
+
+// G = diag(cons) (X + X^T)/2 diag(cons) in FP64
+__global__ void fock_finalize(const unsigned long long* __restrict__ X, const double* __restrict__ cons,
+                              double* __restrict__ G, int n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)n * n) return;
+    const int i = int(k / n), j = int(k % n);
+    const long long s = (long long)(X[(long long)i * n + j] + X[(long long)j * n + i]);
+    G[k] = cons[i] * cons[j] * ((double)s * 0x1p-49);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+using FockLauncher = void (*)(const FockArgs&, const ClassWork&, int, hipStream_t);
+using SchwarzLauncher = void (*)(const int*, int, const int*, const int*, const int*, const int*, const DevQshells&,
+                                 const DevPrims&, const double*, const double*, double*, double*, hipStream_t);
+#define QK_DECL(T) void launch_fock_##T(const FockArgs&, const ClassWork&, int, hipStream_t);
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
+QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
+QK_DECL(dpdp) QK_DECL(ddss) QK_DECL(ddps) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
+#undef QK_DECL
+#define QK_DECL(T) size_t fock_scratch_##T();
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
+QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
+QK_DECL(dpdp) QK_DECL(ddss) QK_DECL(ddps) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
+#undef QK_DECL
+#define QK_DECL(T)                                                                                              \
+    void launch_schwarz_##T(const int*, int, const int*, const int*, const int*, const int*, const DevQshells&, \
+                            const DevPrims&, const double*, const double*, double*, double*, hipStream_t);
+QK_DECL(ss) QK_DECL(ps) QK_DECL(pp) QK_DECL(ds) QK_DECL(dp) QK_DECL(dd)
+#undef QK_DECL
+
+// order matches gen_eri.quartet_classes()
+static const FockLauncher kFockLaunchers[kNumQuartetClasses] = {
+    launch_fock_ssss, launch_fock_psss, launch_fock_psps, launch_fock_ppss, launch_fock_ppps, launch_fock_pppp,
+    launch_fock_dsss, launch_fock_dsps, launch_fock_dspp, launch_fock_dsds, launch_fock_dpss, launch_fock_dpps,
+    launch_fock_dppp, launch_fock_dpds, launch_fock_dpdp, launch_fock_ddss, launch_fock_ddps, launch_fock_ddpp,
+    launch_fock_ddds, launch_fock_dddp, launch_fock_dddd};
+static const SchwarzLauncher kSchwarzLaunchers[kNumPairClasses] = {
+    launch_schwarz_ss, launch_schwarz_ps, launch_schwarz_pp, launch_schwarz_ds, launch_schwarz_dp, launch_schwarz_dd};
+
+using ScratchQuery = size_t (*)();
+static const ScratchQuery kFockScratch[kNumQuartetClasses] = {
+    fock_scratch_ssss, fock_scratch_psss, fock_scratch_psps, fock_scratch_ppss, fock_scratch_ppps, fock_scratch_pppp,
+    fock_scratch_dsss, fock_scratch_dsps, fock_scratch_dspp, fock_scratch_dsds, fock_scratch_dpss, fock_scratch_dpps,
+    fock_scratch_dppp, fock_scratch_dpds, fock_scratch_dpdp, fock_scratch_ddss, fock_scratch_ddps, fock_scratch_ddpp,
+    fock_scratch_ddds, fock_scratch_dddp, fock_scratch_dddd};
+
+void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+    kFockLaunchers[k](a, w, grid, s);
+}
+size_t fock_class_scratch(int k) { return kFockScratch[k](); }
+void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
+                    const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
+                    double* ymax, double* pmax, hipStream_t s) {
+    if (np > 0) kSchwarzLaunchers[pair_class](pairs, np, qa, qb, prim0, nprim, qs, pp, xyz, cons, ymax, pmax, s);
+}
+
+static inline int blocks_for(long long count, int bs) { return int((count + bs - 1) / bs); }
+
+void launch_scale_density(const double* D, const double* cons, double* Dp, int n, hipStream_t s) {
+    hipLaunchKernelGGL(scale_density, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, D, cons, Dp, n);
+}
+void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(density_cutoff, dim3(blocks_for((long long)ns * ns, 256)), dim3(256), 0, s, D, sh_f0, sh_nf,
+                       cm, n, ns);
+}
+void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s) {
+    if (count > 0) hipLaunchKernelGGL(fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
+}
+void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s) {
+    hipLaunchKernelGGL(fock_finalize, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, X, cons, G, n);
+}
+
+}  // namespace qk

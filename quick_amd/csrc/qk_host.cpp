@@ -1,0 +1,885 @@
+// qk_host.cpp -- host side of the MI355X Fock-build library: the context, the
+// class/bound-sorted pair lists (the device work queue), the launches, and the
+// C-ABI (reference gpu_*_ entry points and the explicit qk_* API).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/quick_mi355x.h"
+#include "qk_internal.h"
+#include "eri_flops.h"
+
+namespace qk {
+void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
+size_t fock_class_scratch(int k);
+void launch_scale_density(const double* D, const double* cons, double* Dp, int n, hipStream_t s);
+void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
+                           hipStream_t s);
+void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
+void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s);
+void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
+                    const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
+                    double* ymax, double* pmax, hipStream_t s);
+}  // namespace qk
+
+using namespace qk;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local std::string g_err;
+static void set_err(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+extern "C" const char* qk_last_error(void) { return g_err.c_str(); }
+
+#define HIPCHK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_err("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return -1;                                                                    \
+        }                                                                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffer helper
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        if (count == n && p) return 0;
+        release();
+        if (count == 0) return 0;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            set_err("hipMalloc of %zu bytes failed", count * sizeof(T));
+            return -1;
+        }
+        n = count;
+        return 0;
+    }
+    int upload(const std::vector<T>& v) {
+        if (alloc(v.size())) return -1;
+        if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+            set_err("hipMemcpy H2D failed");
+            return -1;
+        }
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+// ---------------------------------------------------------------------------
+// context
+
+static const int kClassL[kNumQuartetClasses][4] = {
+    {0, 0, 0, 0}, {1, 0, 0, 0}, {1, 0, 1, 0}, {1, 1, 0, 0}, {1, 1, 1, 0}, {1, 1, 1, 1}, {2, 0, 0, 0},
+    {2, 0, 1, 0}, {2, 0, 1, 1}, {2, 0, 2, 0}, {2, 1, 0, 0}, {2, 1, 1, 0}, {2, 1, 1, 1}, {2, 1, 2, 0},
+    {2, 1, 2, 1}, {2, 2, 0, 0}, {2, 2, 1, 0}, {2, 2, 1, 1}, {2, 2, 2, 0}, {2, 2, 2, 1}, {2, 2, 2, 2}};
+static const int kPairL[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+
+static int pair_class_of(int la, int lb) {
+    for (int k = 0; k < kNumPairClasses; ++k)
+        if (kPairL[k][0] == la && kPairL[k][1] == lb) return k;
+    return -1;
+}
+
+struct ClassHost {
+    std::vector<Combo> combos;
+    std::vector<long long> offs;
+    long long total = 0;
+    DBuf<Combo> d_combos;
+    DBuf<long long> d_offs;
+};
+
+struct qk_ctx {
+    int device = 0;
+    // system (host copies, QUICK layouts)
+    int natom = 0, nshell = 0, nbasis = 0, jbasis = 0;
+    std::vector<double> xyz;  // 3 x natom
+    std::vector<int> kstart, kprim, katom, ktype, Qstart, Qfinal, Qsbasis, Qfbasis, ksumtype, KLMN;
+    std::vector<double> cons, gccoeff, gcexpo;
+    bool have_system = false;
+    std::vector<double> Ycut, cutprim;
+    bool have_schwarz = false;
+
+    // Qshells
+    std::vector<int> q_shell, q_L, q_f0, q_atom, q_qf;
+
+    // every Qshell pair (oriented: L(a) >= L(b), equal L: a <= b) and its
+    // primitive pairs; built once per geometry
+    std::vector<int> ap_qa, ap_qb, ap_cls, ap_prim0, ap_nprim;
+    std::vector<int> ap_Pi, ap_Pj;   // global primitive indices, lower shell first (cutprim orientation)
+    std::vector<double> ap_Y;        // Schwarz bound of the parent shell pair
+    DBuf<int> dap_qa, dap_qb, dap_prim0, dap_nprim;
+
+    // prepared work lists and the options they were built for
+    bool prepared = false;
+    double prep_tau = -1, prep_pair_cut = -1;
+    int nclass_pairs = 0;
+    std::vector<ClassHost> classes;
+
+    // device data
+    DBuf<double> d_xyz, d_cons;
+    DBuf<int> dq_atom, dq_f0, dq_shell, dq_qf;
+    DBuf<int> dp_qa, dp_qb, dp_prim0, dp_nprim;
+    DBuf<double> dp_Y;
+    DBuf<double> dpp_zeta, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut;
+    DBuf<int> d_sh_f0, d_sh_nf;
+    DBuf<double> d_Dp, d_cm, d_D, d_G;
+    DBuf<unsigned long long> d_X, d_counters;
+
+    hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_start = nullptr, ev_cls[4] = {nullptr, nullptr, nullptr, nullptr};
+    qk_stats last{};
+    long long cls_quartets[kNumQuartetClasses] = {};
+    long long cls_prims[kNumQuartetClasses] = {};
+    double cls_flops[kNumQuartetClasses] = {};
+    // optional per-class kernel timing (HIP events on the launch stream)
+    bool timing = false;
+    hipEvent_t ev_t0[kNumQuartetClasses] = {}, ev_t1[kNumQuartetClasses] = {};
+    bool cls_timed[kNumQuartetClasses] = {};
+    double cls_ms_sum[kNumQuartetClasses] = {};
+    int cls_ms_n[kNumQuartetClasses] = {};
+
+    // reference-ABI state
+    int method = 0;
+    double abi_tau = 1e-11, abi_prim_limit = 1e-20;
+    std::vector<double> abi_cutmatrix;
+    bool abi_have_cutmatrix = false;
+    bool abi_have_dense = false;
+};
+
+extern "C" void qk_default_opts(qk_opts* o) {
+    o->tau = 1e-11;
+    o->prim_limit = 1e-20;
+    o->pair_cut = 0.0;
+    o->hybrid = 1.0;
+    o->boys_order = QK_BOYS_ORDER_SHELL;
+    o->precision = QK_FP64;
+    o->fp32_threshold = 1e-5;
+}
+
+extern "C" qk_ctx* qk_create(int device) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("no HIP device available");
+        return nullptr;
+    }
+    if (device < 0) device = 0;
+    if (device >= ndev) {
+        set_err("device %d out of range (%d devices)", device, ndev);
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        set_err("hipSetDevice(%d) failed", device);
+        return nullptr;
+    }
+    qk_ctx* c = new qk_ctx();
+    c->device = device;
+    for (auto& s : c->streams)
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            set_err("stream creation failed");
+            delete c;
+            return nullptr;
+        }
+    (void)hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming);
+    for (auto& e : c->ev_cls) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (c->d_counters.alloc(2 * kNumQuartetClasses + 2)) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+extern "C" void qk_destroy(qk_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (auto& s : c->streams)
+        if (s) (void)hipStreamDestroy(s);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        if (c->ev_t0[k]) (void)hipEventDestroy(c->ev_t0[k]);
+        if (c->ev_t1[k]) (void)hipEventDestroy(c->ev_t1[k]);
+    }
+    for (auto& e : c->ev_cls)
+        if (e) (void)hipEventDestroy(e);
+    delete c;
+}
+
+
+// Every Qshell pair and its primitive-pair data (shell.f90:3-88 restated per
+// Qshell pair): zeta = a+b, P = (aA+bB)/(a+b), K = Kpri c_a c_b with
+// Kpri = exp(-ab/(a+b) |AB|^2)/(a+b) and the contraction coefficients QUICK's
+// Xcoeff takes (gccoeff column ksumtype+L, shell.f90:74-80).
+static int build_all_pairs(qk_ctx* c) {
+    const int nq = (int)c->q_shell.size();
+    c->ap_qa.clear(); c->ap_qb.clear(); c->ap_cls.clear(); c->ap_prim0.clear(); c->ap_nprim.clear();
+    c->ap_Pi.clear(); c->ap_Pj.clear();
+    std::vector<double> zeta, Px, Py, Pz, K;
+    for (int i = 0; i < nq; ++i)
+        for (int j = i; j < nq; ++j) {
+            int qa = i, qb = j;
+            if (c->q_L[qa] < c->q_L[qb]) std::swap(qa, qb);
+            const int sa = c->q_shell[qa], sb = c->q_shell[qb];
+            const int La = c->q_L[qa], Lb = c->q_L[qb];
+            c->ap_qa.push_back(qa);
+            c->ap_qb.push_back(qb);
+            c->ap_cls.push_back(pair_class_of(La, Lb));
+            c->ap_prim0.push_back((int)zeta.size());
+            const double* A = &c->xyz[3 * (size_t)(c->katom[sa] - 1)];
+            const double* B = &c->xyz[3 * (size_t)(c->katom[sb] - 1)];
+            const double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
+            const double DAB = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const int ca = c->ksumtype[sa] - 1 + La, cb = c->ksumtype[sb] - 1 + Lb;
+            for (int u = 0; u < c->kprim[sa]; ++u)
+                for (int v = 0; v < c->kprim[sb]; ++v) {
+                    const double a = c->gcexpo[u + 10 * (size_t)ca], b = c->gcexpo[v + 10 * (size_t)cb];
+                    const double kp = std::exp(-a * b / (a + b) * (DAB * DAB)) / (a + b);
+                    zeta.push_back(a + b);
+                    Px.push_back((A[0] * a + B[0] * b) / (a + b));
+                    Py.push_back((A[1] * a + B[1] * b) / (a + b));
+                    Pz.push_back((A[2] * a + B[2] * b) / (a + b));
+                    K.push_back(kp * c->gccoeff[u + 10 * (size_t)ca] * c->gccoeff[v + 10 * (size_t)cb]);
+                    const int Pu = c->kstart[sa] - 1 + u, Pv = c->kstart[sb] - 1 + v;
+                    if (sa <= sb) { c->ap_Pi.push_back(Pu); c->ap_Pj.push_back(Pv); }
+                    else { c->ap_Pi.push_back(Pv); c->ap_Pj.push_back(Pu); }
+                }
+            c->ap_nprim.push_back((int)zeta.size() - c->ap_prim0.back());
+        }
+    c->ap_Y.assign(c->ap_qa.size(), 0.0);
+    (void)hipSetDevice(c->device);
+    if (c->dpp_zeta.upload(zeta) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
+        c->dpp_K.upload(K) || c->dpp_cut.alloc(zeta.size()) || c->dap_qa.upload(c->ap_qa) ||
+        c->dap_qb.upload(c->ap_qb) || c->dap_prim0.upload(c->ap_prim0) || c->dap_nprim.upload(c->ap_nprim))
+        return -1;
+    return 0;
+}
+
+extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell, int nbasis, int jbasis,
+                             const int* kstart, const int* kprim, const int* katom, const int* ktype,
+                             const int* Qstart, const int* Qfinal, const int* Qsbasis, const int* Qfbasis,
+                             const int* ksumtype, const int* KLMN, const double* cons, const double* gccoeff,
+                             const double* gcexpo) {
+    if (!c) { set_err("null context"); return -1; }
+    if (natom <= 0 || nshell <= 0 || nbasis <= 0 || jbasis <= 0) {
+        set_err("invalid sizes natom=%d nshell=%d nbasis=%d jbasis=%d", natom, nshell, nbasis, jbasis);
+        return -2;
+    }
+    c->natom = natom; c->nshell = nshell; c->nbasis = nbasis; c->jbasis = jbasis;
+    c->xyz.assign(xyz, xyz + 3 * (size_t)natom);
+    c->kstart.assign(kstart, kstart + nshell);
+    c->kprim.assign(kprim, kprim + nshell);
+    c->katom.assign(katom, katom + nshell);
+    c->ktype.assign(ktype, ktype + nshell);
+    c->Qstart.assign(Qstart, Qstart + nshell);
+    c->Qfinal.assign(Qfinal, Qfinal + nshell);
+    c->Qsbasis.assign(Qsbasis, Qsbasis + 4 * (size_t)nshell);
+    c->Qfbasis.assign(Qfbasis, Qfbasis + 4 * (size_t)nshell);
+    c->ksumtype.assign(ksumtype, ksumtype + nshell + 1);
+    c->KLMN.assign(KLMN, KLMN + 3 * (size_t)nbasis);
+    c->cons.assign(cons, cons + nbasis);
+    c->gccoeff.assign(gccoeff, gccoeff + 10 * (size_t)nbasis);
+    c->gcexpo.assign(gcexpo, gcexpo + 10 * (size_t)nbasis);
+    // validate what the kernels assume
+    for (int s = 0; s < nshell; ++s) {
+        if (Qstart[s] < 0 || Qfinal[s] > 2 || Qstart[s] > Qfinal[s]) {
+            set_err("shell %d: angular momentum %d..%d outside s/p/d", s + 1, Qstart[s], Qfinal[s]);
+            return -3;
+        }
+        if (kprim[s] < 1 || kprim[s] > 10 || kstart[s] < 1 || kstart[s] + kprim[s] - 1 > jbasis) {
+            set_err("shell %d: primitive range %d+%d outside 1..%d", s + 1, kstart[s], kprim[s], jbasis);
+            return -3;
+        }
+        if (katom[s] < 1 || katom[s] > natom) {
+            set_err("shell %d: atom %d outside 1..%d", s + 1, katom[s], natom);
+            return -3;
+        }
+        if (ksumtype[s] < 1 || ksumtype[s + 1] - 1 > nbasis) {
+            set_err("shell %d: function range outside 1..%d", s + 1, nbasis);
+            return -3;
+        }
+    }
+    // Qshells
+    c->q_shell.clear(); c->q_L.clear(); c->q_f0.clear(); c->q_atom.clear(); c->q_qf.clear();
+    static const int nf_of[3] = {1, 3, 6};
+    for (int s = 0; s < nshell; ++s)
+        for (int L = Qstart[s]; L <= Qfinal[s]; ++L) {
+            const int f0 = ksumtype[s] - 1 + Qsbasis[s + nshell * L];
+            const int f1 = ksumtype[s] - 1 + Qfbasis[s + nshell * L];
+            if (f1 - f0 + 1 != nf_of[L] || f0 < 0 || f1 >= nbasis) {
+                set_err("shell %d type %d: function range %d..%d does not hold %d functions", s + 1, L, f0 + 1,
+                        f1 + 1, nf_of[L]);
+                return -3;
+            }
+            c->q_shell.push_back(s);
+            c->q_L.push_back(L);
+            c->q_f0.push_back(f0);
+            c->q_atom.push_back(katom[s] - 1);
+            c->q_qf.push_back(Qfinal[s]);
+        }
+    (void)hipSetDevice(c->device);
+    std::vector<int> sh_f0(nshell), sh_nf(nshell);
+    for (int s = 0; s < nshell; ++s) {
+        sh_f0[s] = ksumtype[s] - 1 + Qsbasis[s + nshell * Qstart[s]];
+        sh_nf[s] = ksumtype[s] - 1 + Qfbasis[s + nshell * Qfinal[s]] - sh_f0[s] + 1;
+    }
+    if (c->d_xyz.upload(c->xyz) || c->d_cons.upload(c->cons) || c->dq_atom.upload(c->q_atom) ||
+        c->dq_f0.upload(c->q_f0) || c->dq_shell.upload(c->q_shell) || c->dq_qf.upload(c->q_qf) ||
+        c->d_sh_f0.upload(sh_f0) || c->d_sh_nf.upload(sh_nf))
+        return -1;
+    size_t nn = (size_t)nbasis * nbasis;
+    if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
+    if (build_all_pairs(c)) return -1;
+    c->have_system = true;
+    c->have_schwarz = false;
+    c->prepared = false;
+    return 0;
+}
+
+extern "C" int qk_set_schwarz(qk_ctx* c, const double* Ycutoff, const double* cutprim) {
+    if (!c || !c->have_system) { set_err("qk_set_schwarz: no system uploaded"); return -1; }
+    c->Ycut.assign(Ycutoff, Ycutoff + (size_t)c->nshell * c->nshell);
+    c->cutprim.assign(cutprim, cutprim + (size_t)c->jbasis * c->jbasis);
+    const size_t npp = c->ap_Pi.size();
+    std::vector<double> cut(npp);
+    for (size_t k = 0; k < npp; ++k) cut[k] = c->cutprim[c->ap_Pi[k] + (size_t)c->jbasis * c->ap_Pj[k]];
+    for (size_t k = 0; k < c->ap_qa.size(); ++k)
+        c->ap_Y[k] = c->Ycut[c->q_shell[c->ap_qa[k]] + (size_t)c->nshell * c->q_shell[c->ap_qb[k]]];
+    (void)hipSetDevice(c->device);
+    if (c->dpp_cut.upload(cut)) return -1;
+    c->have_schwarz = true;
+    c->prepared = false;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// pair lists and the per-class work queue (host, once per geometry/options)
+
+static int prepare(qk_ctx* c, const qk_opts* o) {
+    if (!c->have_system || !c->have_schwarz) {
+        set_err("prepare: system and Schwarz bounds must be uploaded first");
+        return -1;
+    }
+    if (c->prepared && c->prep_tau == o->tau && c->prep_pair_cut == o->pair_cut) return 0;
+    double Ymax = 0.0;
+    for (double y : c->Ycut) Ymax = std::max(Ymax, y);
+
+    // significant pairs: grouped by pair class, then primitive-pair count
+    // (descending), then Schwarz bound (descending)
+    std::vector<int> sel;
+    for (size_t k = 0; k < c->ap_qa.size(); ++k) {
+        const double y = c->ap_Y[k];
+        if (!(y > o->pair_cut)) continue;
+        if (!(y * Ymax > o->tau)) continue;  // no partner can pass Y_ij*Y_kl > tau
+        sel.push_back((int)k);
+    }
+    std::stable_sort(sel.begin(), sel.end(), [&](int x, int y) {
+        if (c->ap_cls[x] != c->ap_cls[y]) return c->ap_cls[x] < c->ap_cls[y];
+        if (c->ap_nprim[x] != c->ap_nprim[y]) return c->ap_nprim[x] > c->ap_nprim[y];
+        return c->ap_Y[x] > c->ap_Y[y];
+    });
+    const int npair = (int)sel.size();
+    struct Group { int cls, start, count; };
+    std::vector<Group> groups;
+    for (int k = 0; k < npair; ++k) {
+        const int e = sel[k];
+        if (groups.empty() || groups.back().cls != c->ap_cls[e] || c->ap_nprim[sel[groups.back().start]] != c->ap_nprim[e])
+            groups.push_back({c->ap_cls[e], k, 0});
+        groups.back().count++;
+    }
+    std::vector<int> qa(npair), qb(npair), prim0(npair), nprim(npair);
+    std::vector<double> Yp(npair);
+    for (int k = 0; k < npair; ++k) {
+        const int e = sel[k];
+        qa[k] = c->ap_qa[e]; qb[k] = c->ap_qb[e]; prim0[k] = c->ap_prim0[e]; nprim[k] = c->ap_nprim[e];
+        Yp[k] = c->ap_Y[e];
+    }
+
+    // per-class combos (bra group x ket group) with per-bra prefix offsets
+    c->classes.clear();
+    c->classes.resize(kNumQuartetClasses);
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
+        const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
+        ClassHost& CH = c->classes[k];
+        long long base = 0;
+        for (size_t gb = 0; gb < groups.size(); ++gb) {
+            if (groups[gb].cls != cb) continue;
+            for (size_t gk = 0; gk < groups.size(); ++gk) {
+                if (groups[gk].cls != ck) continue;
+                if (cb == ck && gk < gb) continue;
+                const bool same = (gb == gk);
+                const Group& G1 = groups[gb];
+                const Group& G2 = groups[gk];
+                Combo cm;
+                cm.base = base;
+                cm.off_base = (long long)CH.offs.size();
+                cm.bra0 = G1.start;
+                cm.nbra = G1.count;
+                cm.ket0 = G2.start;
+                cm.same = same ? 1 : 0;
+                long long acc = 0;
+                for (int i = 0; i < G1.count; ++i) {
+                    CH.offs.push_back(acc);
+                    const double yp = Yp[G1.start + i];
+                    // kets are sorted by Y descending: first one failing Y_ij*Y_kl > tau
+                    int lo = 0, hi = G2.count;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (yp * Yp[G2.start + mid] > o->tau) lo = mid + 1; else hi = mid;
+                    }
+                    acc += same ? std::max(0, lo - i) : lo;
+                }
+                CH.offs.push_back(acc);
+                if (acc == 0) {
+                    CH.offs.resize(cm.off_base);
+                    continue;
+                }
+                CH.combos.push_back(cm);
+                base += acc;
+            }
+        }
+        CH.total = base;
+    }
+
+    (void)hipSetDevice(c->device);
+    if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_nprim.upload(nprim) ||
+        c->dp_Y.upload(Yp))
+        return -1;
+    for (auto& CH : c->classes)
+        if (CH.d_combos.upload(CH.combos) || CH.d_offs.upload(CH.offs)) return -1;
+    c->nclass_pairs = npair;
+    c->prepared = true;
+    c->prep_tau = o->tau;
+    c->prep_pair_cut = o->pair_cut;
+    return 0;
+}
+
+extern "C" int qk_prepare(qk_ctx* c, const qk_opts* o) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    (void)hipSetDevice(c->device);
+    return prepare(c, o);
+}
+
+// ---------------------------------------------------------------------------
+// Fock build
+
+static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
+    FockArgs a;
+    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_nprim.p, c->dp_Y.p};
+    a.pp = {c->dpp_zeta.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
+    a.xyz = c->d_xyz.p;
+    a.Dp = c->d_Dp.p;
+    a.cutmat = c->d_cm.p;
+    a.X = c->d_X.p;
+    a.counters = c->d_counters.p;
+    a.n = c->nbasis;
+    a.nshell = c->nshell;
+    a.tau = o->tau;
+    a.prim_limit = o->prim_limit;
+    a.hybrid = o->hybrid;
+    a.boys_qshell_order = o->boys_order == QK_BOYS_ORDER_QSHELL ? 1 : 0;
+    return a;
+}
+
+// enqueue the whole build on `s` (classes fan out to the context's streams and join back)
+static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_null, double* dG, const qk_opts* o,
+                        hipStream_t s) {
+    if (prepare(c, o)) return -1;
+    const int n = c->nbasis, ns = c->nshell;
+    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, n, s);
+    if (dcutmat_or_null)
+        HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
+    else
+        launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
+    launch_fill_u64(c->d_X.p, (long long)n * n, s);
+    launch_fill_u64(c->d_counters.p, 2 * kNumQuartetClasses + 2, s);
+    HIPCHK(hipGetLastError());
+    FockArgs a = make_args(c, o);
+    // fold the previous build's kernel times into the running sums (waits for it)
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        if (c->cls_timed[k]) {
+            float ms = 0.0f;
+            HIPCHK(hipEventSynchronize(c->ev_t1[k]));
+            HIPCHK(hipEventElapsedTime(&ms, c->ev_t0[k], c->ev_t1[k]));
+            c->cls_ms_sum[k] += ms;
+            c->cls_ms_n[k] += 1;
+        }
+        c->cls_timed[k] = false;
+    }
+    // heaviest classes first, round-robin over the streams
+    std::vector<int> order(kNumQuartetClasses);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return c->classes[x].total > c->classes[y].total; });
+    // All classes go to the caller's stream, one after another.  Kernels that
+    // spill to scratch get a grid small enough that the private-segment backing
+    // the runtime must provide (per-lane scratch x 64 x resident waves) stays
+    // under kScratchBudget; a persistent grid-stride loop covers the rest.
+    constexpr double kScratchBudget = 256.0 * 1024 * 1024;
+    for (int k : order) {
+        ClassHost& CH = c->classes[k];
+        if (CH.total == 0) continue;
+        ClassWork w{CH.d_combos.p, CH.d_offs.p, (int)CH.combos.size(), CH.total};
+        const long long want = (CH.total + 255) / 256;
+        long long cap = 256LL * 16;
+        const size_t scratch = fock_class_scratch(k);
+        if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+        const int grid = (int)std::min<long long>(want, cap);
+        a.counters = c->d_counters.p + 2 * k;
+        if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
+        launch_fock_class(k, a, w, grid, s);
+        HIPCHK(hipGetLastError());
+        if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
+        c->cls_timed[k] = c->timing;
+    }
+    launch_fock_finalize(c->d_X.p, c->d_cons.p, dG, n, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    if (o->precision != QK_FP64) { set_err("mixed precision is not enabled in this build"); return -4; }
+    (void)hipSetDevice(c->device);
+    return fock_enqueue(c, dD, nullptr, dG, o, (hipStream_t)stream);
+}
+
+// Algorithmic operation count of one class (FMA = 2): per primitive quartet the
+// set-up of P-Q, W and T (35), QUICK's Boys fit plus recursion (24 + 4 LT, exp,
+// sqrt and divisions counted as one each), the prefactor scaling (LT + 1) and the
+// generated VRR; per contracted quartet the screening (12), the generated HRR and
+// the six J/K digestion contractions (12 per integral).
+static double class_flops(int k, long long nq, long long np) {
+    const int* t = kEriFlopTable[k];
+    const int LT = t[0] + t[1] + t[2] + t[3];
+    const double per_prim = 35.0 + (24.0 + 4.0 * LT) + (LT + 1) + t[5];
+    const double per_quartet = 12.0 + t[6] + 12.0 * t[7];
+    return per_prim * double(np) + per_quartet * double(nq);
+}
+
+static int read_stats(qk_ctx* c) {
+    unsigned long long h[2 * kNumQuartetClasses + 2];
+    HIPCHK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
+    c->last = qk_stats{};
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        c->cls_quartets[k] = (long long)h[2 * k];
+        c->cls_prims[k] = (long long)h[2 * k + 1];
+        c->cls_flops[k] = class_flops(k, c->cls_quartets[k], c->cls_prims[k]);
+        c->last.quartets += c->cls_quartets[k];
+        c->last.prim_quartets += c->cls_prims[k];
+        c->last.flops_fp64 += c->cls_flops[k];
+    }
+    return 0;
+}
+
+extern "C" int qk_set_timing(qk_ctx* c, int on) {
+    if (!c) { set_err("null context"); return -1; }
+    (void)hipSetDevice(c->device);
+    if (on && !c->ev_t0[0])
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            HIPCHK(hipEventCreate(&c->ev_t0[k]));
+            HIPCHK(hipEventCreate(&c->ev_t1[k]));
+        }
+    c->timing = on != 0;
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        c->cls_timed[k] = false;
+        c->cls_ms_sum[k] = 0.0;
+        c->cls_ms_n[k] = 0;
+    }
+    return 0;
+}
+
+// per-class figures of the last build: quartets, primitive quartets,
+// algorithmic flops, and (when timing was on) the kernel's elapsed ms
+extern "C" int qk_class_stats(qk_ctx* c, int k, long long* nq, long long* np, double* flops, float* ms) {
+    if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipDeviceSynchronize());
+    if (read_stats(c)) return -1;
+    *nq = c->cls_quartets[k];
+    *np = c->cls_prims[k];
+    *flops = c->cls_flops[k];
+    // average kernel time over every timed build since qk_set_timing(c, 1)
+    double sum = c->cls_ms_sum[k];
+    int cnt = c->cls_ms_n[k];
+    if (c->cls_timed[k]) {
+        float ms1 = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms1, c->ev_t0[k], c->ev_t1[k]));
+        sum += ms1;
+        cnt += 1;
+    }
+    *ms = cnt ? float(sum / cnt) : 0.0f;
+    return 0;
+}
+
+extern "C" int qk_last_stats(qk_ctx* c, qk_stats* s) {
+    if (!c || !s) { set_err("null argument"); return -1; }
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipDeviceSynchronize());
+    if (read_stats(c)) return -1;
+    *s = c->last;
+    return 0;
+}
+
+extern "C" int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    (void)hipSetDevice(c->device);
+    const size_t nn = (size_t)c->nbasis * c->nbasis;
+    if (c->d_D.alloc(nn) || c->d_G.alloc(nn)) return -1;
+    HIPCHK(hipMemcpy(c->d_D.p, D, nn * sizeof(double), hipMemcpyHostToDevice));
+    if (qk_fock_device(c, c->d_D.p, c->d_G.p, o, nullptr)) return -1;
+    std::vector<double> G(nn);
+    HIPCHK(hipMemcpy(G.data(), c->d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nn; ++k) F[k] += G[k];
+    return 0;
+}
+
+extern "C" int qk_num_classes(void) { return kNumQuartetClasses; }
+
+extern "C" int qk_class_info(qk_ctx* c, int k, int* l, long long* work) {
+    if (k < 0 || k >= kNumQuartetClasses) { set_err("class index out of range"); return -1; }
+    for (int i = 0; i < 4; ++i) l[i] = kClassL[k][i];
+    *work = (c && c->prepared) ? c->classes[k].total : 0;
+    return 0;
+}
+
+extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_out) {
+    if (!c || !c->have_system) { set_err("qk_compute_schwarz: no system uploaded"); return -1; }
+    (void)hipSetDevice(c->device);
+    const size_t npairs = c->ap_qa.size(), npp = c->ap_Pi.size();
+    DBuf<double> d_ymax, d_pmax;
+    DBuf<int> d_list;
+    if (d_ymax.alloc(npairs) || d_pmax.alloc(npp)) return -1;
+    std::vector<int> list;
+    std::vector<int> start(kNumPairClasses + 1, 0);
+    for (int cls = 0; cls < kNumPairClasses; ++cls) {
+        start[cls] = (int)list.size();
+        for (size_t k = 0; k < npairs; ++k)
+            if (c->ap_cls[k] == cls) list.push_back((int)k);
+    }
+    start[kNumPairClasses] = (int)list.size();
+    if (d_list.upload(list)) return -1;
+    DevQshells qs{c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    DevPrims pp{c->dpp_zeta.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
+    for (int cls = 0; cls < kNumPairClasses; ++cls) {
+        launch_schwarz(cls, d_list.p + start[cls], start[cls + 1] - start[cls], c->dap_qa.p, c->dap_qb.p,
+                       c->dap_prim0.p, c->dap_nprim.p, qs, pp, c->d_xyz.p, c->d_cons.p, d_ymax.p, d_pmax.p, nullptr);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<double> ymax(npairs), pmax(npp);
+    HIPCHK(hipMemcpy(ymax.data(), d_ymax.p, npairs * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pmax.data(), d_pmax.p, npp * sizeof(double), hipMemcpyDeviceToHost));
+    const int ns = c->nshell, jb = c->jbasis;
+    std::vector<double> Ymx((size_t)ns * ns, 0.0), Pmx((size_t)jb * jb, 0.0);
+    for (size_t k = 0; k < npairs; ++k) {
+        const int sa = c->q_shell[c->ap_qa[k]], sb = c->q_shell[c->ap_qb[k]];
+        const size_t i1 = sa + (size_t)ns * sb, i2 = sb + (size_t)ns * sa;
+        Ymx[i1] = std::max(Ymx[i1], ymax[k]);
+        Ymx[i2] = std::max(Ymx[i2], ymax[k]);
+    }
+    for (size_t k = 0; k < npp; ++k) {
+        const size_t i = c->ap_Pi[k] + (size_t)jb * c->ap_Pj[k];
+        Pmx[i] = std::max(Pmx[i], pmax[k]);
+    }
+    // Ycutoff = sqrt(max |(ij|ij)|), cutprim = sqrt(max |[ab|ab]|) (schwarz.f90:26-27,185)
+    for (auto& v : Ymx) v = std::sqrt(v);
+    // within one shell the reference fills both orientations, and (ab|ab) = (ba|ba):
+    // the cross-type entries of an SP shell land transposed here, so symmetrise
+    for (int s = 0; s < ns; ++s)
+        for (int u = 0; u < c->kprim[s]; ++u)
+            for (int v = u + 1; v < c->kprim[s]; ++v) {
+                const size_t i1 = (c->kstart[s] - 1 + u) + (size_t)jb * (c->kstart[s] - 1 + v);
+                const size_t i2 = (c->kstart[s] - 1 + v) + (size_t)jb * (c->kstart[s] - 1 + u);
+                Pmx[i1] = Pmx[i2] = std::max(Pmx[i1], Pmx[i2]);
+            }
+    std::vector<double> cp((size_t)jb * jb, 0.0);
+    for (size_t k = 0; k < Pmx.size(); ++k) cp[k] = std::sqrt(Pmx[k]);
+    if (Ycut_out) memcpy(Ycut_out, Ymx.data(), Ymx.size() * sizeof(double));
+    if (cutprim_out) memcpy(cutprim_out, cp.data(), cp.size() * sizeof(double));
+    return qk_set_schwarz(c, Ymx.data(), cp.data());
+}
+
+extern "C" int qk_grad_device(qk_ctx* c, const double*, double*, const qk_opts*, void*) {
+    set_err("qk_grad_device: not available yet");
+    (void)c;
+    return -5;
+}
+extern "C" int qk_grad_host(qk_ctx* c, const double*, double*, const qk_opts*) {
+    set_err("qk_grad_host: not available yet");
+    (void)c;
+    return -5;
+}
+
+// ---------------------------------------------------------------------------
+// reference boundary (process-global context, gpu.cu)
+
+static qk_ctx* g_ctx = nullptr;
+static int g_dev_req = -1;
+static bool g_started = false;
+static int g_natom = 0, g_nbasis = 0;
+static std::vector<double> g_xyz;
+static double* g_dense_host = nullptr;
+
+extern "C" qk_ctx* qk_global_context(void) { return g_ctx; }
+
+static void fatal(const char* what) {
+    fprintf(stderr, "quick_mi355x: fatal: %s: %s\n", what, g_err.c_str());
+    fflush(stderr);
+    std::abort();
+}
+
+extern "C" void gpu_startup_(void) { g_started = true; }
+extern "C" void gpu_set_device_(int* id) { g_dev_req = id ? *id : -1; }
+extern "C" void gpu_init_(void) {
+    if (g_ctx) return;
+    g_ctx = qk_create(g_dev_req < 0 ? 0 : g_dev_req);
+    if (!g_ctx) fatal("gpu_init");
+}
+extern "C" void gpu_get_device_info_(int* count, int* id, int* memMB, int* nCU, double* GHz, char* name,
+                                     int* namelen, int* major, int* minor) {
+    int n = 0;
+    (void)hipGetDeviceCount(&n);
+    *count = n;
+    int dev = g_ctx ? g_ctx->device : (g_dev_req < 0 ? 0 : g_dev_req);
+    *id = dev;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        set_err("hipGetDeviceProperties failed");
+        *memMB = 0; *nCU = 0; *GHz = 0; *namelen = 0; *major = 0; *minor = 0;
+        return;
+    }
+    *memMB = (int)(p.totalGlobalMem / (1024 * 1024));
+    *nCU = p.multiProcessorCount;
+    *GHz = p.clockRate * 1e-6;
+    const int L = (int)strnlen(p.gcnArchName, sizeof(p.gcnArchName));
+    memcpy(name, p.gcnArchName, L);
+    *namelen = L;
+    *major = p.major;
+    *minor = p.minor;
+}
+extern "C" void gpu_setup_(int* natom, int* nbasis, int*, int*, int*, int*) {
+    g_natom = *natom;
+    g_nbasis = *nbasis;
+}
+extern "C" void gpu_upload_xyz_(double* xyz) { g_xyz.assign(xyz, xyz + 3 * (size_t)g_natom); }
+extern "C" void gpu_upload_atom_and_chg_(int*, double*) {}
+
+extern "C" void gpu_upload_basis_(int* nshell, int*, int*, int* jbasis, int*, int*, int*, double*, double*, int*,
+                                  int*, int*, int*, int*, int* kstart, int* katom, int* ktype, int* kprim, int*,
+                                  int* Ksumtype, int*, int* Qstart, int* Qfinal, int* Qsbasis, int* Qfbasis,
+                                  double* gccoeff, double* cons, double* gcexpo, int* KLMN) {
+    if (!g_ctx) gpu_init_();
+    if (qk_set_system(g_ctx, g_natom, g_xyz.data(), *nshell, g_nbasis, *jbasis, kstart, kprim, katom, ktype, Qstart,
+                      Qfinal, Qsbasis, Qfbasis, Ksumtype, KLMN, cons, gccoeff, gcexpo))
+        fatal("gpu_upload_basis");
+}
+extern "C" void gpu_upload_cutoff_matrix_(double* Y, double* cutPrim) {
+    if (!g_ctx || qk_set_schwarz(g_ctx, Y, cutPrim)) fatal("gpu_upload_cutoff_matrix");
+}
+extern "C" void gpu_upload_method_(int* m) {
+    if (g_ctx) g_ctx->method = *m;
+}
+extern "C" void gpu_upload_calculated_(double*, double*, double*, double* dense) {
+    if (!g_ctx) fatal("gpu_upload_calculated (no context)");
+    const size_t nn = (size_t)g_ctx->nbasis * g_ctx->nbasis;
+    if (g_ctx->d_D.alloc(nn) || g_ctx->d_G.alloc(nn)) fatal("gpu_upload_calculated");
+    if (hipMemcpy(g_ctx->d_D.p, dense, nn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        set_err("density upload failed");
+        fatal("gpu_upload_calculated");
+    }
+    g_dense_host = dense;
+    g_ctx->abi_have_dense = true;
+}
+extern "C" void gpu_upload_cutoff_(double* cutMatrix, double* integralCutoff, double* primLimit, double*) {
+    if (!g_ctx) fatal("gpu_upload_cutoff (no context)");
+    g_ctx->abi_tau = *integralCutoff;
+    g_ctx->abi_prim_limit = *primLimit;
+    g_ctx->abi_cutmatrix.assign(cutMatrix, cutMatrix + (size_t)g_ctx->nshell * g_ctx->nshell);
+    g_ctx->abi_have_cutmatrix = true;
+}
+extern "C" void gpu_get2e_(double* o) {
+    qk_ctx* c = g_ctx;
+    if (!c || !c->abi_have_dense) {
+        set_err("gpu_get2e: density not uploaded (call gpu_upload_calculated first)");
+        fatal("gpu_get2e");
+    }
+    qk_opts op;
+    qk_default_opts(&op);
+    op.tau = c->abi_tau;
+    op.prim_limit = c->abi_prim_limit;
+    op.hybrid = c->method == 0 ? 1.0 : (c->method == 1 ? 0.2 : 0.0);
+    const size_t nn = (size_t)c->nbasis * c->nbasis;
+    DBuf<double> dcm;
+    const double* cmp = nullptr;
+    if (c->abi_have_cutmatrix) {
+        if (dcm.upload(c->abi_cutmatrix)) fatal("gpu_get2e");
+        cmp = dcm.p;
+    }
+    (void)hipSetDevice(c->device);
+    if (fock_enqueue(c, c->d_D.p, cmp, c->d_G.p, &op, nullptr)) fatal("gpu_get2e");
+    std::vector<double> G(nn);
+    if (hipMemcpy(G.data(), c->d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_err("operator download failed");
+        fatal("gpu_get2e");
+    }
+    for (size_t k = 0; k < nn; ++k) o[k] += G[k];
+    // per-cycle state is consumed by the build (gpu.cu:1857-1861)
+    c->abi_have_dense = false;
+    c->abi_have_cutmatrix = false;
+}
+extern "C" void gpu_upload_grad_(double*, double*) {}
+extern "C" void gpu_grad_(double*) {
+    set_err("gpu_grad: the gradient kernels are not part of this build yet");
+    fatal("gpu_grad");
+}
+extern "C" void gpu_getxc_(int*, double*, double*, double*, double*, double*) {
+    set_err("gpu_getxc: the XC step is not provided by this library");
+}
+extern "C" void gpu_aoint_(double*, double*, int*, char*) {
+    set_err("gpu_aoint: stored-integral (NODIRECT) mode is not provided by this library");
+}
+extern "C" void gpu_addint_(double*, int*, char*) {
+    set_err("gpu_addint: stored-integral (NODIRECT) mode is not provided by this library");
+}
+extern "C" void gpu_cleanup_(void) {
+    if (g_ctx) {
+        g_ctx->prepared = false;
+        g_ctx->have_schwarz = false;
+    }
+}
+extern "C" void gpu_shutdown_(void) {
+    qk_destroy(g_ctx);
+    g_ctx = nullptr;
+    g_started = false;
+}

@@ -1,0 +1,375 @@
+// qk_kernels.h -- device code shared by the per-class translation units.
+#pragma once
+#include "qk_internal.h"
+#include "eri_generated.inc"
+
+namespace qk {
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int nrange(int lo, int hi) {
+    return hi < lo ? 0 : ncart(hi) + nrange(lo, hi - 1);
+}
+
+// QUICK's FmT (subs/fmt.f90:12-84; gpu_get2e_subs.h:1386-1457) restated: the
+// piecewise fit of F_0 (or F_Mb near T = 0), then upward recursion for
+// T > 3e-5 (T > 3e-7 when Mb < 4), else downward recursion from Mb.  Mb is the
+// order the reference asks for; values F_0..F_LT are produced (LT <= Mb).
+template <int LT>
+__device__ __forceinline__ void boys_ref(double X, int Mb, double* F) {
+    const double PIE4 = 3.1415926535897932384626433832795 / 4.0;
+    const double E = exp(-X);
+    const bool up = (X > 3.0e-5) || (X > 3.0e-7 && Mb < 4);
+    if (up) {
+        const double XINV = 1.0 / X;
+        double WW1;
+        if (X > 5.0) {
+            const double s = sqrt(PIE4 * XINV);
+            if (X > 33.0) {
+                WW1 = s;
+            } else if (X > 15.0) {
+                WW1 = ((1.9623264149430e-01 * XINV - 4.9695241464490e-01) * XINV - 6.0156581186481e-05) * E + s;
+            } else if (X > 10.0) {
+                WW1 = (((-1.8784686463512e-01 * XINV + 2.2991849164985e-01) * XINV - 4.9893752514047e-01) * XINV -
+                       2.1916512131607e-05) * E + s;
+            } else {
+                WW1 = ((((((4.6897511375022e-01 * XINV - 6.9955602298985e-01) * XINV + 5.3689283271887e-01) * XINV -
+                          3.2883030418398e-01) * XINV + 2.4645596956002e-01) * XINV - 4.9984072848436e-01) * XINV -
+                       3.1501078774085e-06) * E + s;
+            }
+        } else if (X > 1.0) {
+            double F1;
+            if (X > 3.0) {
+                const double Y = X - 4.0;
+                F1 = ((((((((((-2.62453564772299e-11 * Y + 3.24031041623823e-10) * Y - 3.614965656163e-09) * Y +
+                             3.760256799971e-08) * Y - 3.553558319675e-07) * Y + 3.022556449731e-06) * Y -
+                          2.290098979647e-05) * Y + 1.526537461148e-04) * Y - 8.81947375894379e-04) * Y +
+                       4.33207949514611e-03) * Y - 1.75257821619926e-02) * Y + 5.28406320615584e-02;
+            } else {
+                const double Y = X - 2.0;
+                F1 = ((((((((((-1.61702782425558e-10 * Y + 1.96215250865776e-09) * Y - 2.14234468198419e-08) * Y +
+                             2.17216556336318e-07) * Y - 1.98850171329371e-06) * Y + 1.62429321438911e-05) * Y -
+                          1.16740298039895e-04) * Y + 7.24888732052332e-04) * Y - 3.79490003707156e-03) * Y +
+                       1.61723488664661e-02) * Y - 5.29428148329736e-02) * Y + 1.15702180856167e-01;
+            }
+            WW1 = (X + X) * F1 + E;
+        } else {
+            const double F1 = ((((((((-8.36313918003957e-08 * X + 1.21222603512827e-06) * X - 1.15662609053481e-05) * X +
+                                    9.25197374512647e-05) * X - 6.40994113129432e-04) * X + 3.78787044215009e-03) * X -
+                                 1.85185172458485e-02) * X + 7.14285713298222e-02) * X - 1.99999999997023e-01) * X +
+                              3.33333333333318e-01;
+            WW1 = (X + X) * F1 + E;
+        }
+        F[0] = WW1;
+#pragma unroll
+        for (int m = 1; m <= LT; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
+    } else {
+        double v = (1.0 - X) / double(2 * Mb + 1);
+        const double twot = X * 2.0;
+        for (int m = Mb - 1; m >= LT; --m) v = (twot * v + E) / double(m * 2 + 1);
+        F[LT] = v;
+#pragma unroll
+        for (int m = LT - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) / double(m * 2 + 1);
+    }
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long* X, long long idx, double v) {
+    const long long iv = __double2ll_rn(v * 0x1p48);
+    if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
+}
+
+__device__ __forceinline__ double dn_max(const double* cm, int ns, int A, int B, int C, int D) {
+    double m = fmax(4.0 * cm[A * ns + B], 4.0 * cm[C * ns + D]);
+    m = fmax(m, fmax(fmax(cm[A * ns + D], cm[A * ns + C]), fmax(cm[B * ns + C], cm[B * ns + D])));
+    return m;
+}
+
+// flat index -> (bra pair, ket pair)
+__device__ __forceinline__ void decode(const ClassWork& w, long long gid, int& p, int& q) {
+    int lo = 0, hi = w.ncombo - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (w.combos[mid].base <= gid) lo = mid; else hi = mid - 1;
+    }
+    const Combo cb = w.combos[lo];
+    const long long local = gid - cb.base;
+    const long long* off = w.offs + cb.off_base;
+    int a = 0, b = cb.nbra - 1;
+    while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (off[mid] <= local) a = mid; else b = mid - 1;
+    }
+    p = cb.bra0 + a;
+    q = cb.ket0 + (cb.same ? a : 0) + int(local - off[a]);
+}
+
+template <int LA, int LB, int LC, int LD>
+struct ClassOps;
+
+#define QK_CLASS(la, lb, lc, ld, TAG)                                                                    \
+    template <>                                                                                          \
+    struct ClassOps<la, lb, lc, ld> {                                                                    \
+        template <typename R>                                                                            \
+        static __device__ __forceinline__ void vrr(const R* F, R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, \
+                                                   R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, R oo2z,     \
+                                                   R oo2e, R oo2ze, R roz, R roe, R* acc) {              \
+            vrr_##TAG<R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, oo2z, oo2e,      \
+                         oo2ze, roz, roe, acc);                                                          \
+        }                                                                                                \
+        template <typename R>                                                                            \
+        static __device__ __forceinline__ void hrr(const R* acc, R ABx, R ABy, R ABz, R CDx, R CDy,      \
+                                                   R CDz, R* out) {                                      \
+            hrr_##TAG<R>(acc, ABx, ABy, ABz, CDx, CDy, CDz, out);                                        \
+        }                                                                                                \
+    };
+
+QK_CLASS(0, 0, 0, 0, ssss)
+QK_CLASS(1, 0, 0, 0, psss)
+QK_CLASS(1, 0, 1, 0, psps)
+QK_CLASS(1, 1, 0, 0, ppss)
+QK_CLASS(1, 1, 1, 0, ppps)
+QK_CLASS(1, 1, 1, 1, pppp)
+QK_CLASS(2, 0, 0, 0, dsss)
+QK_CLASS(2, 0, 1, 0, dsps)
+QK_CLASS(2, 0, 1, 1, dspp)
+QK_CLASS(2, 0, 2, 0, dsds)
+QK_CLASS(2, 1, 0, 0, dpss)
+QK_CLASS(2, 1, 1, 0, dpps)
+QK_CLASS(2, 1, 1, 1, dppp)
+QK_CLASS(2, 1, 2, 0, dpds)
+QK_CLASS(2, 1, 2, 1, dpdp)
+QK_CLASS(2, 2, 0, 0, ddss)
+QK_CLASS(2, 2, 1, 0, ddps)
+QK_CLASS(2, 2, 1, 1, ddpp)
+QK_CLASS(2, 2, 2, 0, ddds)
+QK_CLASS(2, 2, 2, 1, dddp)
+QK_CLASS(2, 2, 2, 2, dddd)
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256) fock_kernel(const FockArgs a, const ClassWork w) {
+    constexpr int NE = nrange(LA, LA + LB), NF = nrange(LC, LC + LD);
+    constexpr int NACC = NE * NF;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NOUT = NA * NB * NC * ND;
+    constexpr int LT = LA + LB + LC + LD;
+    const double X0 = 34.98683665524972;  // 2 pi^(5/2), quick_constants_module.f90:55
+    const int n = a.n;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned long long nsurv = 0, nprimq = 0;
+
+    for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < w.total; gid += stride) {
+        int p, q;
+        decode(w, gid, p, q);
+        const int qa = a.pr.qa[p], qb = a.pr.qb[p], qc = a.pr.qa[q], qd = a.pr.qb[q];
+        const int sA = a.q.shell[qa], sB = a.q.shell[qb], sC = a.q.shell[qc], sD = a.q.shell[qd];
+        const double dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
+        if (!(a.pr.Y[q] * a.pr.Y[p] * dn > a.tau)) continue;
+        ++nsurv;
+        const int Mb = a.boys_qshell_order ? LT : a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd];
+
+        const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
+        const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
+        const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+
+        double acc[NACC];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+
+        const int pb0 = a.pr.prim0[p], nbp = a.pr.nprim[p];
+        const int pk0 = a.pr.prim0[q], nkp = a.pr.nprim[q];
+        for (int i = 0; i < nbp; ++i) {
+            const double zab = a.pp.zeta[pb0 + i];
+            const double Px = a.pp.Px[pb0 + i], Py = a.pp.Py[pb0 + i], Pz = a.pp.Pz[pb0 + i];
+            const double X2 = X0 * a.pp.K[pb0 + i];
+            const double cp1 = dn * a.pp.cut[pb0 + i];
+            for (int j = 0; j < nkp; ++j) {
+                if (!(cp1 * a.pp.cut[pk0 + j] > a.prim_limit)) continue;
+                ++nprimq;
+                const double zcd = a.pp.zeta[pk0 + j];
+                const double Qx = a.pp.Px[pk0 + j], Qy = a.pp.Py[pk0 + j], Qz = a.pp.Pz[pk0 + j];
+                const double ABCD = zab + zcd;
+                const double ROU = zab * zcd / ABCD;
+                const double Wx = (Px * zab + Qx * zcd) / ABCD;
+                const double Wy = (Py * zab + Qy * zcd) / ABCD;
+                const double Wz = (Pz * zab + Qz * zcd) / ABCD;
+                const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                double F[LT + 1];
+                boys_ref<LT>(T, Mb, F);
+                const double s = X2 * a.pp.K[pk0 + j] / sqrt(ABCD);
+#pragma unroll
+                for (int m = 0; m <= LT; ++m) F[m] *= s;
+                ClassOps<LA, LB, LC, LD>::template vrr<double>(
+                    F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
+                    Wx - Qx, Wy - Qy, Wz - Qz, 0.5 / zab, 0.5 / zcd, 0.5 / ABCD, zcd / ABCD, zab / ABCD, acc);
+            }
+        }
+
+        double out[NOUT];
+        {
+            const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
+            const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+            ClassOps<LA, LB, LC, LD>::template hrr<double>(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
+                                                           Cz - Dz, out);
+        }
+
+        // digestion: G = (X + X^T)/2 with X accumulating
+        //   X[a][b] += 4w D[c][d] I,  X[c][d] += 4w D[a][b] I,
+        //   X[a][c] -= h w D[b][d] I, X[a][d] -= h w D[b][c] I,
+        //   X[b][c] -= h w D[a][d] I, X[b][d] -= h w D[a][c] I
+        // w = 1/((1+[qa==qb])(1+[qc==qd])(1+[p==q])) is the share of the quartet's
+        // 8-fold orbit this component set represents.
+        const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
+        double wgt = 1.0;
+        if (qa == qb) wgt *= 0.5;
+        if (qc == qd) wgt *= 0.5;
+        if (p == q) wgt *= 0.5;
+        const double* Dp = a.Dp;
+        const double jw = 4.0 * wgt, kw = -a.hybrid * wgt;
+#define OUT(ia, ib, ic, id) out[(((ia)*NB + (ib)) * NC + (ic)) * ND + (id)]
+        for (int ia = 0; ia < NA; ++ia)
+            for (int ib = 0; ib < NB; ++ib) {
+                double s = 0.0;
+                for (int ic = 0; ic < NC; ++ic)
+                    for (int id = 0; id < ND; ++id) s += Dp[(long long)(fc + ic) * n + fd + id] * OUT(ia, ib, ic, id);
+                fx_add(a.X, (long long)(fa + ia) * n + fb + ib, jw * s);
+            }
+        for (int ic = 0; ic < NC; ++ic)
+            for (int id = 0; id < ND; ++id) {
+                double s = 0.0;
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
+                fx_add(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+            }
+        if (a.hybrid != 0.0) {
+            for (int ia = 0; ia < NA; ++ia)
+                for (int ic = 0; ic < NC; ++ic) {
+                    double s = 0.0;
+                    for (int ib = 0; ib < NB; ++ib)
+                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
+                    fx_add(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
+                }
+            for (int ia = 0; ia < NA; ++ia)
+                for (int id = 0; id < ND; ++id) {
+                    double s = 0.0;
+                    for (int ib = 0; ib < NB; ++ib)
+                        for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
+                    fx_add(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
+                }
+            for (int ib = 0; ib < NB; ++ib)
+                for (int ic = 0; ic < NC; ++ic) {
+                    double s = 0.0;
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
+                    fx_add(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
+                }
+            for (int ib = 0; ib < NB; ++ib)
+                for (int id = 0; id < ND; ++id) {
+                    double s = 0.0;
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
+                    fx_add(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
+                }
+        }
+#undef OUT
+    }
+    if (nsurv) atomicAdd(a.counters + 0, nsurv);
+    if (nprimq) atomicAdd(a.counters + 1, nprimq);
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz bounds (schwarz.f90:46-548): for every Qshell pair of one pair class,
+//   ymax[p]        = max over components of |(ab|ab)| contracted over all
+//                    primitive quartets (classcutoff), and
+//   pmax[prim i]   = max over components of |[ab|ab]| for the primitive pair i
+//                    with itself, weighted by (X0 Xcoeff_i)^2 (classprim).
+// The host reduces both over the Qshells of a shell pair and takes sqrt.
+template <int LA, int LB>
+__global__ void __launch_bounds__(256) schwarz_kernel(const int* __restrict__ pairs, int npairs, const int* __restrict__ pqa,
+                                                      const int* __restrict__ pqb, const int* __restrict__ prim0,
+                                                      const int* __restrict__ nprim, const DevQshells qs,
+                                                      const DevPrims pp, const double* __restrict__ xyz,
+                                                      const double* __restrict__ cons,
+                                                      double* __restrict__ ymax, double* __restrict__ pmax) {
+    constexpr int NE = nrange(LA, LA + LB);
+    constexpr int NACC = NE * NE;
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    constexpr int NOUT = NA * NB * NA * NB;
+    constexpr int LT = 2 * (LA + LB);
+    const double X0 = 34.98683665524972;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    const int p = pairs[t];
+    const int qa = pqa[p], qb = pqb[p];
+    const int Mb = 2 * (qs.qfinal[qa] + qs.qfinal[qb]);
+    const int atA = qs.atom[qa], atB = qs.atom[qb];
+    const double Ax = xyz[3 * atA], Ay = xyz[3 * atA + 1], Az = xyz[3 * atA + 2];
+    const double ABx = Ax - xyz[3 * atB], ABy = Ay - xyz[3 * atB + 1], ABz = Az - xyz[3 * atB + 2];
+    const int p0 = prim0[p], np = nprim[p];
+    double acc[NACC];
+    double out[NOUT];
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+    for (int i = 0; i < np; ++i) {
+        const double zab = pp.zeta[p0 + i];
+        const double Px = pp.Px[p0 + i], Py = pp.Py[p0 + i], Pz = pp.Pz[p0 + i];
+        const double X2 = X0 * pp.K[p0 + i];
+        for (int j = 0; j < np; ++j) {
+            const double zcd = pp.zeta[p0 + j];
+            const double Qx = pp.Px[p0 + j], Qy = pp.Py[p0 + j], Qz = pp.Pz[p0 + j];
+            const double ABCD = zab + zcd;
+            const double ROU = zab * zcd / ABCD;
+            const double Wx = (Px * zab + Qx * zcd) / ABCD, Wy = (Py * zab + Qy * zcd) / ABCD,
+                         Wz = (Pz * zab + Qz * zcd) / ABCD;
+            const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+            const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+            double F[LT + 1];
+            boys_ref<LT>(T, Mb, F);
+            const double s = X2 * pp.K[p0 + j] / sqrt(ABCD);
+#pragma unroll
+            for (int m = 0; m <= LT; ++m) F[m] *= s;
+            ClassOps<LA, LB, LA, LB>::template vrr<double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz,
+                                                           Qx - Ax, Qy - Ay, Qz - Az, Wx - Qx, Wy - Qy, Wz - Qz,
+                                                           0.5 / zab, 0.5 / zcd, 0.5 / ABCD, zcd / ABCD, zab / ABCD, acc);
+        }
+    }
+    ClassOps<LA, LB, LA, LB>::template hrr<double>(acc, ABx, ABy, ABz, ABx, ABy, ABz, out);
+    // (ab|ab) carries cons(a)^2 cons(b)^2 (hrrwhole's cons product, subs/hrr.f90:295)
+    double cab[NA * NB];
+    for (int a = 0; a < NA; ++a)
+        for (int b = 0; b < NB; ++b) {
+            const double ca = cons[qs.f0[qa] + a], cb = cons[qs.f0[qb] + b];
+            cab[a * NB + b] = (ca * ca) * (cb * cb);
+        }
+    double m = 0.0;
+    for (int a = 0; a < NA; ++a)
+        for (int b = 0; b < NB; ++b) m = fmax(m, fabs(out[((a * NB + b) * NA + a) * NB + b] * cab[a * NB + b]));
+    ymax[p] = m;
+    // one primitive pair with itself: P = Q, T = 0
+    for (int i = 0; i < np; ++i) {
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+        const double zab = pp.zeta[p0 + i];
+        const double Px = pp.Px[p0 + i], Py = pp.Py[p0 + i], Pz = pp.Pz[p0 + i];
+        const double ABCD = zab + zab;
+        const double Wx = (Px * zab + Px * zab) / ABCD, Wy = (Py * zab + Py * zab) / ABCD,
+                     Wz = (Pz * zab + Pz * zab) / ABCD;
+        double F[LT + 1];
+        boys_ref<LT>(0.0, Mb, F);
+        const double x4 = X0 * pp.K[p0 + i];
+        const double s = 1.0 / sqrt(ABCD);
+#pragma unroll
+        for (int k = 0; k <= LT; ++k) F[k] *= s;
+        ClassOps<LA, LB, LA, LB>::template vrr<double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz,
+                                                       Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz,
+                                                       0.5 / zab, 0.5 / zab, 0.5 / ABCD, zab / ABCD, zab / ABCD, acc);
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) acc[k] *= x4 * x4;
+        ClassOps<LA, LB, LA, LB>::template hrr<double>(acc, ABx, ABy, ABz, ABx, ABy, ABz, out);
+        double mp = 0.0;
+        for (int a = 0; a < NA; ++a)
+            for (int b = 0; b < NB; ++b) mp = fmax(mp, fabs(out[((a * NB + b) * NA + a) * NB + b] * cab[a * NB + b]));
+        pmax[p0 + i] = mp;
+    }
+}
+
+}  // namespace qk

@@ -156,6 +156,23 @@ def small_system(name: str) -> tuple[list[str], np.ndarray]:
     raise KeyError(name)
 
 
+def synthetic_density(xyz_bohr, katom, ksumtype, nbasis, seed):
+    """Seeded symmetric density whose entries decay as exp(-0.25 R^2) with the
+    distance R (bohr) between the two functions' centres, so that QUICK's
+    density screening (max |D| over shell blocks) prunes distant quartets the
+    way a converged density does."""
+    rng = np.random.default_rng(seed)
+    centre = np.zeros(nbasis, np.int64)
+    for s in range(len(katom)):
+        centre[ksumtype[s] - 1:ksumtype[s + 1] - 1] = katom[s] - 1
+    r = np.asarray(xyz_bohr)[:, centre].T
+    R2 = ((r[:, None, :] - r[None, :, :]) ** 2).sum(-1)
+    u = rng.uniform(-1.0, 1.0, (nbasis, nbasis))
+    D = 0.6 * np.exp(-0.25 * R2) * (u + u.T) / 2.0
+    D[np.diag_indices(nbasis)] += 0.5 + 0.5 * rng.random(nbasis)
+    return D
+
+
 def to_quick_input(symbols, xyz_ang, keywords: str) -> str:
     """A QUICK input deck (keyword card, blank line, atoms in Angstrom)."""
     lines = [keywords, ""]

@@ -1,0 +1,180 @@
+"""HIP Fock build vs the oracle and the reference's golden vectors (MI355X)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from quick_amd.basis import build_basis
+from quick_amd.fock import FockEngine, make_opts, QK_BOYS_ORDER_QSHELL
+from quick_amd.geometry import CONFIGS, small_system
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = ["h2o", "ch4", "water2", "ethane", "water4"]
+TOL_F = 1e-10      # |dF_ij|, Eh (FP64 parity bar)
+TOL_E = 1e-8       # |dE|, Eh
+
+
+def _banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_fock_matches_reference_cpu_path(golden, name):
+    _banner(f"fock parity {name}")
+    fx = golden(name)
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G = eng.fock(fx.D)
+    dG = np.abs(G - fx.G).max()
+    dE = abs(0.5 * np.sum(fx.D * (G - fx.G)))
+    assert dG <= TOL_F, dG
+    assert dE <= TOL_E, dE
+    assert np.array_equal(G, G.T)
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_schwarz_on_device_matches_reference(golden, name):
+    _banner(f"schwarz {name}")
+    fx = golden(name)
+    eng = FockEngine(fx)
+    Y, cp = eng.compute_schwarz()
+    assert np.abs(Y - fx.Ycutoff).max() <= 1e-12
+    assert np.abs(cp - fx.cutprim).max() <= 1e-12
+
+
+def test_reference_gpu_screening_mode_matches_oracle(golden):
+    _banner("reference-GPU screening mode water4")
+    fx = golden("water4")
+    o = Oracle(fx)
+    o.set_cutoffs(fx.Ycutoff, fx.cutprim)
+    Gref, _, _ = o.fock(fx.D, mode="gpu", pair_cut=1e-10)
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G = eng.fock(fx.D, make_opts(pair_cut=1e-10, boys_order=QK_BOYS_ORDER_QSHELL))
+    assert np.abs(G - Gref).max() <= TOL_F
+
+
+def test_bitwise_deterministic(golden):
+    _banner("determinism water4")
+    fx = golden("water4")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G1 = eng.fock(fx.D)
+    G2 = eng.fock(fx.D)
+    assert np.array_equal(G1, G2)
+
+
+def test_device_pointer_entry_point(golden):
+    import torch
+    _banner("device entry point water2")
+    fx = golden("water2")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    D = torch.from_numpy(np.asfortranarray(fx.D).ravel(order="F").copy()).cuda()
+    G = torch.empty_like(D)
+    eng.fock_device(D.data_ptr(), G.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    Gh = G.cpu().numpy().reshape(fx.nbasis, fx.nbasis, order="F")
+    assert np.abs(Gh - fx.G).max() <= TOL_F
+
+
+def test_hybrid_exchange_weight(golden):
+    _banner("hybrid weight water2")
+    fx = golden("water2")
+    o = Oracle(fx)
+    o.set_cutoffs(fx.Ycutoff, fx.cutprim)
+    Gref, _, _ = o.fock(fx.D, mode="gpu_cpuboys", pair_cut=0.0, hybrid=0.2)
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G = eng.fock(fx.D, make_opts(hybrid=0.2))
+    assert np.abs(G - Gref).max() <= TOL_F
+
+
+def _water32_engine():
+    syms, xyz = CONFIGS["water32"]()
+    b = build_basis(syms, xyz)
+    eng = FockEngine(b)
+    eng.compute_schwarz()
+    return b, eng
+
+
+def _density(b, seed=3):
+    from quick_amd.geometry import synthetic_density
+    return synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, seed)
+
+
+def test_water32_translation_and_permutation_invariance():
+    """Full-size checks that need no oracle: translating the cluster leaves G
+    unchanged; reversing the order of the waters permutes G accordingly."""
+    _banner("water32 invariances")
+    syms, xyz = CONFIGS["water32"]()
+    b = build_basis(syms, xyz)
+    D = _density(b)
+    e1 = FockEngine(b)
+    e1.compute_schwarz()
+    G1 = e1.fock(D)
+    assert np.array_equal(G1, G1.T)
+    e1.close()
+    b2 = build_basis(syms, xyz + np.array([1.25, -0.5, 2.0]))
+    e2 = FockEngine(b2)
+    e2.compute_schwarz()
+    G2 = e2.fock(D)
+    assert np.abs(G1 - G2).max() <= 1e-9
+    e2.close()
+    nw = len(syms) // 3
+    order = np.concatenate([[3 * w, 3 * w + 1, 3 * w + 2] for w in reversed(range(nw))])
+    b3 = build_basis([syms[i] for i in order], xyz[order])
+    per_w = b.nbasis // nw
+    fperm = np.concatenate([np.arange(w * per_w, (w + 1) * per_w) for w in reversed(range(nw))])
+    D3 = D[np.ix_(fperm, fperm)]
+    e3 = FockEngine(b3)
+    e3.compute_schwarz()
+    G3 = e3.fock(D3)
+    assert np.abs(G3 - G1[np.ix_(fperm, fperm)]).max() <= 1e-9
+
+
+def test_abi_reference_call_sequence(golden):
+    """The Fortran call sequence of main.f90/hfoperator.f90 through the gpu_*_ symbols."""
+    from quick_amd.fock import lib
+    _banner("reference ABI h2o")
+    fx = golden("h2o")
+    L = lib()
+    i = lambda v: ctypes.byref(ctypes.c_int(v))
+    d = lambda v: ctypes.byref(ctypes.c_double(v))
+    arr = lambda a, t: np.ascontiguousarray(np.asarray(a, t).ravel(order="F"))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    L.gpu_startup_()
+    L.gpu_set_device_(i(-1))
+    L.gpu_init_()
+    L.gpu_setup_(i(int(fx.natom)), i(int(fx.nbasis)), i(10), i(1), i(0), i(0))
+    xyz = arr(fx.xyz, np.float64)
+    L.gpu_upload_xyz_(P(xyz))
+    ints = {k: arr(getattr(fx, k), np.int32) for k in ("kstart", "katom", "ktype", "kprim", "ksumtype",
+                                                       "Qstart", "Qfinal", "Qsbasis", "Qfbasis", "KLMN")}
+    dbl = {k: arr(getattr(fx, k), np.float64) for k in ("gccoeff", "cons", "gcexpo")}
+    z = np.zeros(16, np.int32)
+    zd = np.zeros(16)
+    L.gpu_upload_basis_(i(int(fx.nshell)), i(int(fx.jbasis)), i(int(fx.nshell)), i(int(fx.jbasis)), i(10),
+                        P(z), P(z), P(zd), P(zd), P(z), P(z), P(z), P(z), P(z),
+                        P(ints["kstart"]), P(ints["katom"]), P(ints["ktype"]), P(ints["kprim"]), P(z),
+                        P(ints["ksumtype"]), P(z), P(ints["Qstart"]), P(ints["Qfinal"]), P(ints["Qsbasis"]),
+                        P(ints["Qfbasis"]), P(dbl["gccoeff"]), P(dbl["cons"]), P(dbl["gcexpo"]), P(ints["KLMN"]))
+    Yc, cp = arr(fx.Ycutoff, np.float64), arr(fx.cutprim, np.float64)
+    L.gpu_upload_cutoff_matrix_(P(Yc), P(cp))
+    o = Oracle(fx)
+    cm = arr(o.density_cutoff(fx.D), np.float64)
+    H = np.random.default_rng(5).normal(size=(fx.nbasis, fx.nbasis))
+    H = H + H.T
+    O = arr(H, np.float64)
+    dense = arr(fx.D, np.float64)
+    L.gpu_upload_method_(i(0))
+    L.gpu_upload_calculated_(P(O), P(zd), P(zd), P(dense))
+    L.gpu_upload_cutoff_(P(cm), d(1e-11), d(1e-20))
+    L.gpu_get2e_(P(O))
+    F = O.reshape(fx.nbasis, fx.nbasis, order="F")
+    assert np.abs(F - (H + fx.G)).max() <= TOL_F
+    L.gpu_cleanup_()
+    L.gpu_shutdown_()
