@@ -146,7 +146,7 @@ struct qk_ctx {
     DBuf<int> dq_atom, dq_f0, dq_shell, dq_qf;
     DBuf<int> dp_qa, dp_qb, dp_prim0, dp_nprim;
     DBuf<double> dp_Y;
-    DBuf<double> dpp_zeta, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut;
+    DBuf<double> dpp_zeta, dpp_oo2z, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut;
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
@@ -271,8 +271,10 @@ static int build_all_pairs(qk_ctx* c) {
             c->ap_nprim.push_back((int)zeta.size() - c->ap_prim0.back());
         }
     c->ap_Y.assign(c->ap_qa.size(), 0.0);
+    std::vector<double> oo2z(zeta.size());
+    for (size_t k = 0; k < zeta.size(); ++k) oo2z[k] = 0.5 / zeta[k];
     (void)hipSetDevice(c->device);
-    if (c->dpp_zeta.upload(zeta) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
+    if (c->dpp_zeta.upload(zeta) || c->dpp_oo2z.upload(oo2z) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
         c->dpp_K.upload(K) || c->dpp_cut.alloc(zeta.size()) || c->dap_qa.upload(c->ap_qa) ||
         c->dap_qb.upload(c->ap_qb) || c->dap_prim0.upload(c->ap_prim0) || c->dap_nprim.upload(c->ap_nprim))
         return -1;
@@ -494,7 +496,7 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     FockArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
     a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_nprim.p, c->dp_Y.p};
-    a.pp = {c->dpp_zeta.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
+    a.pp = {c->dpp_zeta.p, c->dpp_oo2z.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
     a.xyz = c->d_xyz.p;
     a.Dp = c->d_Dp.p;
     a.cutmat = c->d_cm.p;
@@ -691,7 +693,7 @@ extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_o
     start[kNumPairClasses] = (int)list.size();
     if (d_list.upload(list)) return -1;
     DevQshells qs{c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
-    DevPrims pp{c->dpp_zeta.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
+    DevPrims pp{c->dpp_zeta.p, c->dpp_oo2z.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
     for (int cls = 0; cls < kNumPairClasses; ++cls) {
         launch_schwarz(cls, d_list.p + start[cls], start[cls + 1] - start[cls], c->dap_qa.p, c->dap_qb.p,
                        c->dap_prim0.p, c->dap_nprim.p, qs, pp, c->d_xyz.p, c->d_cons.p, d_ymax.p, d_pmax.p, nullptr);

@@ -33,6 +33,7 @@ struct DevPairs {
 // Per primitive pair of a significant Qshell pair.
 struct DevPrims {
     double* zeta;  // a + b
+    double* oo2z;  // 0.5 / (a + b)
     double* Px;
     double* Py;
     double* Pz;

@@ -144,8 +144,23 @@ QK_CLASS(2, 2, 2, 0, ddds)
 QK_CLASS(2, 2, 2, 1, dddp)
 QK_CLASS(2, 2, 2, 2, dddd)
 
+// Blocks of 256 threads kept resident per CU: light classes are pushed to a
+// higher occupancy (fewer VGPRs), the heavy d classes keep the full register file.
 template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256) fock_kernel(const FockArgs a, const ClassWork w) {
+struct Occupancy {
+    static constexpr int nacc = nrange(LA, LA + LB) * nrange(LC, LC + LD);
+    static constexpr int value = nacc <= 9 ? 4 : (nacc <= 36 ? 2 : 1);
+};
+
+// int64 sum over the 64 lanes of a wavefront (exact, so order-free)
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_kernel(const FockArgs a, const ClassWork w) {
     constexpr int NE = nrange(LA, LA + LB), NF = nrange(LC, LC + LD);
     constexpr int NACC = NE * NF;
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -153,124 +168,158 @@ __global__ void __launch_bounds__(256) fock_kernel(const FockArgs a, const Class
     constexpr int LT = LA + LB + LC + LD;
     const double X0 = 34.98683665524972;  // 2 pi^(5/2), quick_constants_module.f90:55
     const int n = a.n;
-    const long long stride = (long long)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    const long long wstride = (long long)gridDim.x * blockDim.x;
     unsigned long long nsurv = 0, nprimq = 0;
 
-    for (long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x; gid < w.total; gid += stride) {
-        int p, q;
-        decode(w, gid, p, q);
-        const int qa = a.pr.qa[p], qb = a.pr.qb[p], qc = a.pr.qa[q], qd = a.pr.qb[q];
-        const int sA = a.q.shell[qa], sB = a.q.shell[qb], sC = a.q.shell[qc], sD = a.q.shell[qd];
-        const double dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
-        if (!(a.pr.Y[q] * a.pr.Y[p] * dn > a.tau)) continue;
-        ++nsurv;
-        const int Mb = a.boys_qshell_order ? LT : a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd];
+    // wave-uniform loop: every lane of a wavefront reaches the reductions below
+    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x - lane); base < w.total; base += wstride) {
+        const long long gid = base + lane;
+        const bool inrange = gid < w.total;
+        int p = 0, q = 0;
+        if (inrange) decode(w, gid, p, q);
+        const int p0 = __shfl(p, 0, 64);
+        // lanes of one wavefront walk consecutive kets, mostly of one bra pair
+        const bool bra_uniform = __all(!inrange || p == p0);
 
-        const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
-        const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
-        const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
-
-        double acc[NACC];
-#pragma unroll
-        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
-
-        const int pb0 = a.pr.prim0[p], nbp = a.pr.nprim[p];
-        const int pk0 = a.pr.prim0[q], nkp = a.pr.nprim[q];
-        for (int i = 0; i < nbp; ++i) {
-            const double zab = a.pp.zeta[pb0 + i];
-            const double Px = a.pp.Px[pb0 + i], Py = a.pp.Py[pb0 + i], Pz = a.pp.Pz[pb0 + i];
-            const double X2 = X0 * a.pp.K[pb0 + i];
-            const double cp1 = dn * a.pp.cut[pb0 + i];
-            for (int j = 0; j < nkp; ++j) {
-                if (!(cp1 * a.pp.cut[pk0 + j] > a.prim_limit)) continue;
-                ++nprimq;
-                const double zcd = a.pp.zeta[pk0 + j];
-                const double Qx = a.pp.Px[pk0 + j], Qy = a.pp.Py[pk0 + j], Qz = a.pp.Pz[pk0 + j];
-                const double ABCD = zab + zcd;
-                const double ROU = zab * zcd / ABCD;
-                const double Wx = (Px * zab + Qx * zcd) / ABCD;
-                const double Wy = (Py * zab + Qy * zcd) / ABCD;
-                const double Wz = (Pz * zab + Qz * zcd) / ABCD;
-                const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                double F[LT + 1];
-                boys_ref<LT>(T, Mb, F);
-                const double s = X2 * a.pp.K[pk0 + j] / sqrt(ABCD);
-#pragma unroll
-                for (int m = 0; m <= LT; ++m) F[m] *= s;
-                ClassOps<LA, LB, LC, LD>::template vrr<double>(
-                    F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
-                    Wx - Qx, Wy - Qy, Wz - Qz, 0.5 / zab, 0.5 / zcd, 0.5 / ABCD, zcd / ABCD, zab / ABCD, acc);
-            }
+        const int qa = a.pr.qa[p], qb = a.pr.qb[p];
+        const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
+        bool active = false;
+        double dn = 0.0;
+        if (inrange) {
+            const int sA = a.q.shell[qa], sB = a.q.shell[qb], sC = a.q.shell[qc], sD = a.q.shell[qd];
+            dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
+            active = a.pr.Y[q] * a.pr.Y[p] * dn > a.tau;
         }
-
-        double out[NOUT];
-        {
-            const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
-            const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
-            ClassOps<LA, LB, LC, LD>::template hrr<double>(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
-                                                           Cz - Dz, out);
-        }
-
-        // digestion: G = (X + X^T)/2 with X accumulating
-        //   X[a][b] += 4w D[c][d] I,  X[c][d] += 4w D[a][b] I,
-        //   X[a][c] -= h w D[b][d] I, X[a][d] -= h w D[b][c] I,
-        //   X[b][c] -= h w D[a][d] I, X[b][d] -= h w D[a][c] I
-        // w = 1/((1+[qa==qb])(1+[qc==qd])(1+[p==q])) is the share of the quartet's
-        // 8-fold orbit this component set represents.
-        const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
-        double wgt = 1.0;
-        if (qa == qb) wgt *= 0.5;
-        if (qc == qd) wgt *= 0.5;
-        if (p == q) wgt *= 0.5;
+        const int fa = a.q.f0[qa], fb = a.q.f0[qb];
         const double* Dp = a.Dp;
-        const double jw = 4.0 * wgt, kw = -a.hybrid * wgt;
+        double jab[NA * NB];
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
+
+        if (active) {
+            ++nsurv;
+            const int Mb = a.boys_qshell_order ? LT : a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd];
+            const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
+            const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
+            const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+
+            double acc[NACC];
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+
+            const int pb0 = a.pr.prim0[p], nbp = a.pr.nprim[p];
+            const int pk0 = a.pr.prim0[q], nkp = a.pr.nprim[q];
+            for (int i = 0; i < nbp; ++i) {
+                const double zab = a.pp.zeta[pb0 + i], oo2z = a.pp.oo2z[pb0 + i];
+                const double Px = a.pp.Px[pb0 + i], Py = a.pp.Py[pb0 + i], Pz = a.pp.Pz[pb0 + i];
+                const double X2 = X0 * a.pp.K[pb0 + i];
+                const double cp1 = dn * a.pp.cut[pb0 + i];
+                for (int j = 0; j < nkp; ++j) {
+                    if (!(cp1 * a.pp.cut[pk0 + j] > a.prim_limit)) continue;
+                    ++nprimq;
+                    const double zcd = a.pp.zeta[pk0 + j];
+                    const double Qx = a.pp.Px[pk0 + j], Qy = a.pp.Py[pk0 + j], Qz = a.pp.Pz[pk0 + j];
+                    const double inv = 1.0 / (zab + zcd);   // the one division per primitive quartet
+                    const double roz = zcd * inv, roe = zab * inv;
+                    const double ROU = zab * roz;
+                    const double Wx = (Px * zab + Qx * zcd) * inv;
+                    const double Wy = (Py * zab + Qy * zcd) * inv;
+                    const double Wz = (Pz * zab + Qz * zcd) * inv;
+                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    double F[LT + 1];
+                    boys_ref<LT>(T, Mb, F);
+                    const double s = X2 * a.pp.K[pk0 + j] * sqrt(inv);
+#pragma unroll
+                    for (int m = 0; m <= LT; ++m) F[m] *= s;
+                    ClassOps<LA, LB, LC, LD>::template vrr<double>(
+                        F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
+                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.pp.oo2z[pk0 + j], 0.5 * inv, roz, roe, acc);
+                }
+            }
+
+            double out[NOUT];
+            {
+                const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
+                const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+                ClassOps<LA, LB, LC, LD>::template hrr<double>(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
+                                                               Cz - Dz, out);
+            }
+
+            // digestion: G = (X + X^T)/2 with X accumulating
+            //   X[a][b] += 4w D[c][d] I,  X[c][d] += 4w D[a][b] I,
+            //   X[a][c] -= h w D[b][d] I, X[a][d] -= h w D[b][c] I,
+            //   X[b][c] -= h w D[a][d] I, X[b][d] -= h w D[a][c] I
+            // w = 1/((1+[qa==qb])(1+[qc==qd])(1+[p==q])) is the share of the quartet's
+            // 8-fold orbit this component set represents.
+            const int fc = a.q.f0[qc], fd = a.q.f0[qd];
+            double wgt = 1.0;
+            if (qa == qb) wgt *= 0.5;
+            if (qc == qd) wgt *= 0.5;
+            if (p == q) wgt *= 0.5;
+            const double jw = 4.0 * wgt, kw = -a.hybrid * wgt;
 #define OUT(ia, ib, ic, id) out[(((ia)*NB + (ib)) * NC + (ic)) * ND + (id)]
-        for (int ia = 0; ia < NA; ++ia)
-            for (int ib = 0; ib < NB; ++ib) {
-                double s = 0.0;
-                for (int ic = 0; ic < NC; ++ic)
-                    for (int id = 0; id < ND; ++id) s += Dp[(long long)(fc + ic) * n + fd + id] * OUT(ia, ib, ic, id);
-                fx_add(a.X, (long long)(fa + ia) * n + fb + ib, jw * s);
-            }
-        for (int ic = 0; ic < NC; ++ic)
-            for (int id = 0; id < ND; ++id) {
-                double s = 0.0;
+            for (int ia = 0; ia < NA; ++ia)
+                for (int ib = 0; ib < NB; ++ib) {
+                    double s = 0.0;
+                    for (int ic = 0; ic < NC; ++ic)
+                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fc + ic) * n + fd + id] * OUT(ia, ib, ic, id);
+                    jab[ia * NB + ib] = jw * s;
+                }
+            for (int ic = 0; ic < NC; ++ic)
+                for (int id = 0; id < ND; ++id) {
+                    double s = 0.0;
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
+                    fx_add(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+                }
+            if (a.hybrid != 0.0) {
                 for (int ia = 0; ia < NA; ++ia)
-                    for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
-                fx_add(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+                    for (int ic = 0; ic < NC; ++ic) {
+                        double s = 0.0;
+                        for (int ib = 0; ib < NB; ++ib)
+                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
+                        fx_add(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
+                    }
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int id = 0; id < ND; ++id) {
+                        double s = 0.0;
+                        for (int ib = 0; ib < NB; ++ib)
+                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
+                        fx_add(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
+                    }
+                for (int ib = 0; ib < NB; ++ib)
+                    for (int ic = 0; ic < NC; ++ic) {
+                        double s = 0.0;
+                        for (int ia = 0; ia < NA; ++ia)
+                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
+                        fx_add(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
+                    }
+                for (int ib = 0; ib < NB; ++ib)
+                    for (int id = 0; id < ND; ++id) {
+                        double s = 0.0;
+                        for (int ia = 0; ia < NA; ++ia)
+                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
+                        fx_add(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
+                    }
             }
-        if (a.hybrid != 0.0) {
-            for (int ia = 0; ia < NA; ++ia)
-                for (int ic = 0; ic < NC; ++ic) {
-                    double s = 0.0;
-                    for (int ib = 0; ib < NB; ++ib)
-                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                    fx_add(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
-                }
-            for (int ia = 0; ia < NA; ++ia)
-                for (int id = 0; id < ND; ++id) {
-                    double s = 0.0;
-                    for (int ib = 0; ib < NB; ++ib)
-                        for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                    fx_add(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
-                }
-            for (int ib = 0; ib < NB; ++ib)
-                for (int ic = 0; ic < NC; ++ic) {
-                    double s = 0.0;
-                    for (int ia = 0; ia < NA; ++ia)
-                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                    fx_add(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
-                }
-            for (int ib = 0; ib < NB; ++ib)
-                for (int id = 0; id < ND; ++id) {
-                    double s = 0.0;
-                    for (int ia = 0; ia < NA; ++ia)
-                        for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                    fx_add(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
-                }
-        }
 #undef OUT
+        }
+
+        // J on the bra block: converted to fixed point per lane, summed across the
+        // wavefront when every lane shares the bra pair (one atomic per element
+        // instead of up to 64), else added lane by lane.
+        if (bra_uniform) {
+#pragma unroll
+            for (int k = 0; k < NA * NB; ++k) {
+                const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
+                if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            }
+        } else if (active) {
+#pragma unroll
+            for (int k = 0; k < NA * NB; ++k) fx_add(a.X, (long long)(fa + k / NB) * n + fb + k % NB, jab[k]);
+        }
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);
     if (nprimq) atomicAdd(a.counters + 1, nprimq);

@@ -178,3 +178,27 @@ def test_abi_reference_call_sequence(golden):
     assert np.abs(F - (H + fx.G)).max() <= TOL_F
     L.gpu_cleanup_()
     L.gpu_shutdown_()
+
+
+def test_water32_full_size_matches_reference_samples(golden):
+    """(H2O)32, 800 bf: the benchmark system against the reference CPU path's
+    own full build (40k seeded element samples, diagonal, row norms, E2)."""
+    from quick_amd.geometry import synthetic_density
+    _banner("water32 vs reference samples")
+    fx = golden("water32")
+    syms, xyz = CONFIGS["water32"]()
+    b = build_basis(syms, xyz)
+    b.xyz = fx.xyz.copy()                      # the coordinates exactly as the reference read them
+    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, int(fx.density_seed))
+    eng = FockEngine(b)
+    Y, _ = eng.compute_schwarz()
+    assert np.abs(Y - fx.Ycutoff).max() <= 1e-12
+    G = eng.fock(D)
+    dg = np.abs(G[fx.G_sample_i, fx.G_sample_j] - fx.G_sample).max()
+    dd = np.abs(np.diag(G) - fx.G_diag).max()
+    dn = np.abs(np.linalg.norm(G, axis=1) - fx.G_rownorm).max()
+    dE = abs(0.5 * np.sum(D * G) - float(fx.E2))
+    print(f"water32: max|dG| samples {dg:.2e} diag {dd:.2e} rownorm {dn:.2e} |dE2| {dE:.2e}")
+    assert dg <= TOL_F and dd <= TOL_F
+    assert dn <= 1e-9
+    assert dE <= TOL_E
