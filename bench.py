@@ -134,13 +134,13 @@ def main():
     cls = eng.class_stats()
     st = eng.stats()
     dom = max(cls, key=lambda c: c[4])
-    tag, nq, npq, flops, kms = dom
+    tag, nq, npq, flops, kms, _ = dom
     achieved = flops / (kms * 1e-3) / 1e12
     total_flops = sum(c[3] for c in cls)
     for c in sorted(cls, key=lambda c: -c[4]):
         if c[1]:
             log(f"  class {c[0]}: {c[1]:>10d} quartets {c[2]:>11d} prim {c[4]:8.3f} ms "
-                f"{c[3] / max(c[4], 1e-9) / 1e9:8.2f} TFLOP/s")
+                f"{c[3] / max(c[4], 1e-9) / 1e9:8.2f} TFLOP/s  lane fill {c[5]:.2f}")
     Gh = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
     e2 = 0.5 * float(np.sum(D * Gh))
 

@@ -152,6 +152,8 @@ int qk_last_stats(qk_ctx* c, qk_stats* s);
  * (la,lb,lc,ld, quartet-space size) of class k. */
 int qk_num_classes(void);
 int qk_class_info(qk_ctx* c, int k, int* lalblcld, long long* work);
+/* Fraction of wavefront lanes the work tiles of class k occupy (0..1). */
+double qk_class_lane_fill(qk_ctx* c, int k);
 /* Record HIP events around every class kernel of the following builds (on the
  * stream the kernels run on). */
 int qk_set_timing(qk_ctx* c, int on);

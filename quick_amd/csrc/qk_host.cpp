@@ -107,11 +107,9 @@ static int pair_class_of(int la, int lb) {
 }
 
 struct ClassHost {
-    std::vector<Combo> combos;
-    std::vector<long long> offs;
-    long long total = 0;
-    DBuf<Combo> d_combos;
-    DBuf<long long> d_offs;
+    std::vector<Tile> tiles;
+    long long total = 0;   // quartets passing the Schwarz test
+    DBuf<Tile> d_tiles;
 };
 
 struct qk_ctx {
@@ -133,7 +131,11 @@ struct qk_ctx {
     std::vector<int> ap_qa, ap_qb, ap_cls, ap_prim0, ap_nprim;
     std::vector<int> ap_Pi, ap_Pj;   // global primitive indices, lower shell first (cutprim orientation)
     std::vector<double> ap_Y;        // Schwarz bound of the parent shell pair
+    std::vector<double> ap_zeta, ap_oo2z, ap_Px, ap_Py, ap_Pz, ap_K, ap_cut;  // per primitive pair
     DBuf<int> dap_qa, dap_qb, dap_prim0, dap_nprim;
+    // the kernels' primitive-major copy of the significant pairs' primitive pairs
+    DBuf<int> dp_pstride;
+    DBuf<double> dk_zeta, dk_oo2z, dk_Px, dk_Py, dk_Pz, dk_K, dk_cut;
 
     // prepared work lists and the options they were built for
     bool prepared = false;
@@ -273,6 +275,8 @@ static int build_all_pairs(qk_ctx* c) {
     c->ap_Y.assign(c->ap_qa.size(), 0.0);
     std::vector<double> oo2z(zeta.size());
     for (size_t k = 0; k < zeta.size(); ++k) oo2z[k] = 0.5 / zeta[k];
+    c->ap_zeta = zeta; c->ap_oo2z = oo2z; c->ap_Px = Px; c->ap_Py = Py; c->ap_Pz = Pz; c->ap_K = K;
+    c->ap_cut.assign(zeta.size(), 0.0);
     (void)hipSetDevice(c->device);
     if (c->dpp_zeta.upload(zeta) || c->dpp_oo2z.upload(oo2z) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
         c->dpp_K.upload(K) || c->dpp_cut.alloc(zeta.size()) || c->dap_qa.upload(c->ap_qa) ||
@@ -369,6 +373,7 @@ extern "C" int qk_set_schwarz(qk_ctx* c, const double* Ycutoff, const double* cu
     const size_t npp = c->ap_Pi.size();
     std::vector<double> cut(npp);
     for (size_t k = 0; k < npp; ++k) cut[k] = c->cutprim[c->ap_Pi[k] + (size_t)c->jbasis * c->ap_Pj[k]];
+    c->ap_cut = cut;
     for (size_t k = 0; k < c->ap_qa.size(); ++k)
         c->ap_Y[k] = c->Ycut[c->q_shell[c->ap_qa[k]] + (size_t)c->nshell * c->q_shell[c->ap_qb[k]]];
     (void)hipSetDevice(c->device);
@@ -413,22 +418,41 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
             groups.push_back({c->ap_cls[e], k, 0});
         groups.back().count++;
     }
-    std::vector<int> qa(npair), qb(npair), prim0(npair), nprim(npair);
+    std::vector<int> qa(npair), qb(npair), prim0(npair), pstride(npair), nprim(npair);
     std::vector<double> Yp(npair);
     for (int k = 0; k < npair; ++k) {
         const int e = sel[k];
-        qa[k] = c->ap_qa[e]; qb[k] = c->ap_qb[e]; prim0[k] = c->ap_prim0[e]; nprim[k] = c->ap_nprim[e];
+        qa[k] = c->ap_qa[e]; qb[k] = c->ap_qb[e]; nprim[k] = c->ap_nprim[e];
         Yp[k] = c->ap_Y[e];
     }
+    // primitive-major copy per group: entry (pair k, primitive j) at
+    // base(group) + j * count(group) + (k - start(group))
+    std::vector<double> kz, ko, kx, ky, kzz, kK, kc;
+    for (const Group& g : groups) {
+        const int base = (int)kz.size();
+        const int np = c->ap_nprim[sel[g.start]];
+        for (int j = 0; j < np; ++j)
+            for (int k = g.start; k < g.start + g.count; ++k) {
+                const int src = c->ap_prim0[sel[k]] + j;
+                kz.push_back(c->ap_zeta[src]); ko.push_back(c->ap_oo2z[src]);
+                kx.push_back(c->ap_Px[src]); ky.push_back(c->ap_Py[src]); kzz.push_back(c->ap_Pz[src]);
+                kK.push_back(c->ap_K[src]); kc.push_back(c->ap_cut[src]);
+            }
+        for (int k = g.start; k < g.start + g.count; ++k) {
+            prim0[k] = base + (k - g.start);
+            pstride[k] = g.count;
+        }
+    }
 
-    // per-class combos (bra group x ket group) with per-bra prefix offsets
+    // per-class tiles: for every (bra group, ket group) block and every bra,
+    // the prefix of kets passing Y_ij*Y_kl > tau cut into runs of <= 64
     c->classes.clear();
     c->classes.resize(kNumQuartetClasses);
     for (int k = 0; k < kNumQuartetClasses; ++k) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         ClassHost& CH = c->classes[k];
-        long long base = 0;
+        long long total = 0;
         for (size_t gb = 0; gb < groups.size(); ++gb) {
             if (groups[gb].cls != cb) continue;
             for (size_t gk = 0; gk < groups.size(); ++gk) {
@@ -437,16 +461,7 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                 const bool same = (gb == gk);
                 const Group& G1 = groups[gb];
                 const Group& G2 = groups[gk];
-                Combo cm;
-                cm.base = base;
-                cm.off_base = (long long)CH.offs.size();
-                cm.bra0 = G1.start;
-                cm.nbra = G1.count;
-                cm.ket0 = G2.start;
-                cm.same = same ? 1 : 0;
-                long long acc = 0;
                 for (int i = 0; i < G1.count; ++i) {
-                    CH.offs.push_back(acc);
                     const double yp = Yp[G1.start + i];
                     // kets are sorted by Y descending: first one failing Y_ij*Y_kl > tau
                     int lo = 0, hi = G2.count;
@@ -454,26 +469,24 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                         const int mid = (lo + hi) >> 1;
                         if (yp * Yp[G2.start + mid] > o->tau) lo = mid + 1; else hi = mid;
                     }
-                    acc += same ? std::max(0, lo - i) : lo;
+                    const int q_begin = same ? i : 0;
+                    for (int q = q_begin; q < lo; q += 64)
+                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(64, lo - q), 0});
+                    if (lo > q_begin) total += lo - q_begin;
                 }
-                CH.offs.push_back(acc);
-                if (acc == 0) {
-                    CH.offs.resize(cm.off_base);
-                    continue;
-                }
-                CH.combos.push_back(cm);
-                base += acc;
             }
         }
-        CH.total = base;
+        CH.total = total;
     }
 
     (void)hipSetDevice(c->device);
-    if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_nprim.upload(nprim) ||
-        c->dp_Y.upload(Yp))
+    if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_pstride.upload(pstride) ||
+        c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dk_zeta.upload(kz) || c->dk_oo2z.upload(ko) ||
+        c->dk_Px.upload(kx) || c->dk_Py.upload(ky) || c->dk_Pz.upload(kzz) || c->dk_K.upload(kK) ||
+        c->dk_cut.upload(kc))
         return -1;
     for (auto& CH : c->classes)
-        if (CH.d_combos.upload(CH.combos) || CH.d_offs.upload(CH.offs)) return -1;
+        if (CH.d_tiles.upload(CH.tiles)) return -1;
     c->nclass_pairs = npair;
     c->prepared = true;
     c->prep_tau = o->tau;
@@ -495,8 +508,8 @@ extern "C" int qk_prepare(qk_ctx* c, const qk_opts* o) {
 static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     FockArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
-    a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_nprim.p, c->dp_Y.p};
-    a.pp = {c->dpp_zeta.p, c->dpp_oo2z.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
+    a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
+    a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
     a.xyz = c->d_xyz.p;
     a.Dp = c->d_Dp.p;
     a.cutmat = c->d_cm.p;
@@ -540,16 +553,17 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     std::vector<int> order(kNumQuartetClasses);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int x, int y) { return c->classes[x].total > c->classes[y].total; });
-    // All classes go to the caller's stream, one after another.  Kernels that
-    // spill to scratch get a grid small enough that the private-segment backing
-    // the runtime must provide (per-lane scratch x 64 x resident waves) stays
-    // under kScratchBudget; a persistent grid-stride loop covers the rest.
+    // All classes run one after another on the caller's stream (one hardware
+    // queue: kernels that spill to scratch never run concurrently with others).
+    // Kernels that spill get a grid small enough that their private-segment
+    // backing (per-lane scratch x 64 x resident waves) stays under
+    // kScratchBudget; the grid-stride loop over tiles covers the rest.
     constexpr double kScratchBudget = 256.0 * 1024 * 1024;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
-        if (CH.total == 0) continue;
-        ClassWork w{CH.d_combos.p, CH.d_offs.p, (int)CH.combos.size(), CH.total};
-        const long long want = (CH.total + 255) / 256;
+        if (CH.tiles.empty()) continue;
+        ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size()};
+        const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
         long long cap = 256LL * 16;
         const size_t scratch = fock_class_scratch(k);
         if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
@@ -674,6 +688,12 @@ extern "C" int qk_class_info(qk_ctx* c, int k, int* l, long long* work) {
     for (int i = 0; i < 4; ++i) l[i] = kClassL[k][i];
     *work = (c && c->prepared) ? c->classes[k].total : 0;
     return 0;
+}
+
+// lanes occupied by the tiles of class k: quartets / (64 * tiles)
+extern "C" double qk_class_lane_fill(qk_ctx* c, int k) {
+    if (!c || !c->prepared || k < 0 || k >= kNumQuartetClasses || c->classes[k].tiles.empty()) return 0.0;
+    return double(c->classes[k].total) / (64.0 * double(c->classes[k].tiles.size()));
 }
 
 extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_out) {

@@ -25,12 +25,16 @@ struct DevQshells {
 struct DevPairs {
     int* qa;
     int* qb;
-    int* prim0;    // first primitive pair
+    int* prim0;    // index of the pair's first primitive pair
+    int* pstride;  // distance between its consecutive primitive pairs
     int* nprim;    // number of primitive pairs
     double* Y;     // Schwarz bound of the parent shell pair
 };
 
-// Per primitive pair of a significant Qshell pair.
+// Per primitive pair.  In the kernels' copy the primitive pairs of a group of
+// pairs with equal primitive count are stored primitive-major: entry j of the
+// pairs q, q+1, ... of a group is contiguous, so the lanes of a wavefront (one
+// ket pair each) load primitive j of their kets with coalesced accesses.
 struct DevPrims {
     double* zeta;  // a + b
     double* oo2z;  // 0.5 / (a + b)
@@ -41,28 +45,27 @@ struct DevPrims {
     double* cut;   // QUICK cutprim of the primitive pair
 };
 
-// One (bra group x ket group) block of a quartet class.  Bra and ket groups are
-// runs of pairs of one pair class with equal primitive-pair count, sorted by Y
-// descending, so the kets surviving Y_p*Y_q > tau form a prefix for every bra.
-struct Combo {
-    long long base;      // first quartet index of the block within its class
-    long long off_base;  // start of this block's per-bra offsets in the offsets array
-    int bra0, nbra;      // bra pairs [bra0, bra0+nbra)
-    int ket0;            // first ket pair
-    int same;            // bra group == ket group: kets start at the bra itself
+// A wavefront's unit of work: one bra pair p and up to 64 consecutive ket pairs
+// q0 .. q0+len-1 of one ket group (one per lane).  Bra and ket groups are runs
+// of pairs of one pair class with equal primitive-pair count, sorted by Schwarz
+// bound descending, so the kets passing Y_p*Y_q > tau form a prefix of the ket
+// group for every bra; the host cuts those prefixes into tiles.
+struct Tile {
+    int p;
+    int q0;
+    int len;
+    int pad;
 };
 
 struct ClassWork {
-    const Combo* combos;
-    const long long* offs;  // per-bra exclusive prefix sums (nbra+1 per combo)
-    int ncombo;
-    long long total;        // quartets in the class's search space
+    const Tile* tiles;
+    long long ntiles;
 };
 
 struct FockArgs {
     DevQshells q;
     DevPairs pr;
-    DevPrims pp;
+    DevPrims pp;    // primitive-major per group (see DevPrims)
     const double* xyz;        // 3 x natom
     const double* Dp;         // cons-scaled density, n x n
     const double* cutmat;     // nshell x nshell, max |D| over shell blocks

@@ -83,25 +83,6 @@ __device__ __forceinline__ double dn_max(const double* cm, int ns, int A, int B,
     return m;
 }
 
-// flat index -> (bra pair, ket pair)
-__device__ __forceinline__ void decode(const ClassWork& w, long long gid, int& p, int& q) {
-    int lo = 0, hi = w.ncombo - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (w.combos[mid].base <= gid) lo = mid; else hi = mid - 1;
-    }
-    const Combo cb = w.combos[lo];
-    const long long local = gid - cb.base;
-    const long long* off = w.offs + cb.off_base;
-    int a = 0, b = cb.nbra - 1;
-    while (a < b) {
-        const int mid = (a + b + 1) >> 1;
-        if (off[mid] <= local) a = mid; else b = mid - 1;
-    }
-    p = cb.bra0 + a;
-    q = cb.ket0 + (cb.same ? a : 0) + int(local - off[a]);
-}
-
 template <int LA, int LB, int LC, int LD>
 struct ClassOps;
 
@@ -169,18 +150,17 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const double X0 = 34.98683665524972;  // 2 pi^(5/2), quick_constants_module.f90:55
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    const long long wstride = (long long)gridDim.x * blockDim.x;
+    const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
 
-    // wave-uniform loop: every lane of a wavefront reaches the reductions below
-    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x - lane); base < w.total; base += wstride) {
-        const long long gid = base + lane;
-        const bool inrange = gid < w.total;
-        int p = 0, q = 0;
-        if (inrange) decode(w, gid, p, q);
-        const int p0 = __shfl(p, 0, 64);
-        // lanes of one wavefront walk consecutive kets, mostly of one bra pair
-        const bool bra_uniform = __all(!inrange || p == p0);
+    // one tile per wavefront and iteration: a bra pair shared by all lanes and
+    // one ket pair per lane; every lane reaches the reduction below
+    for (long long t = wave0; t < w.ntiles; t += nwaves) {
+        const Tile tl = w.tiles[t];
+        const int p = __builtin_amdgcn_readfirstlane(tl.p);
+        const int q = tl.q0 + lane;
+        const bool inrange = lane < tl.len;
 
         const int qa = a.pr.qa[p], qb = a.pr.qb[p];
         const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
@@ -208,18 +188,20 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #pragma unroll
             for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
-            const int pb0 = a.pr.prim0[p], nbp = a.pr.nprim[p];
-            const int pk0 = a.pr.prim0[q], nkp = a.pr.nprim[q];
+            const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+            const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
             for (int i = 0; i < nbp; ++i) {
-                const double zab = a.pp.zeta[pb0 + i], oo2z = a.pp.oo2z[pb0 + i];
-                const double Px = a.pp.Px[pb0 + i], Py = a.pp.Py[pb0 + i], Pz = a.pp.Pz[pb0 + i];
-                const double X2 = X0 * a.pp.K[pb0 + i];
-                const double cp1 = dn * a.pp.cut[pb0 + i];
+                const int bi = pb0 + i * pbs;
+                const double zab = a.pp.zeta[bi], oo2z = a.pp.oo2z[bi];
+                const double Px = a.pp.Px[bi], Py = a.pp.Py[bi], Pz = a.pp.Pz[bi];
+                const double X2 = X0 * a.pp.K[bi];
+                const double cp1 = dn * a.pp.cut[bi];
                 for (int j = 0; j < nkp; ++j) {
-                    if (!(cp1 * a.pp.cut[pk0 + j] > a.prim_limit)) continue;
+                    const int kj = pk0 + j * pks;
+                    if (!(cp1 * a.pp.cut[kj] > a.prim_limit)) continue;
                     ++nprimq;
-                    const double zcd = a.pp.zeta[pk0 + j];
-                    const double Qx = a.pp.Px[pk0 + j], Qy = a.pp.Py[pk0 + j], Qz = a.pp.Pz[pk0 + j];
+                    const double zcd = a.pp.zeta[kj];
+                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
                     const double inv = 1.0 / (zab + zcd);   // the one division per primitive quartet
                     const double roz = zcd * inv, roe = zab * inv;
                     const double ROU = zab * roz;
@@ -230,12 +212,12 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                     const double T = (dx * dx + dy * dy + dz * dz) * ROU;
                     double F[LT + 1];
                     boys_ref<LT>(T, Mb, F);
-                    const double s = X2 * a.pp.K[pk0 + j] * sqrt(inv);
+                    const double s = X2 * a.pp.K[kj] * sqrt(inv);
 #pragma unroll
                     for (int m = 0; m <= LT; ++m) F[m] *= s;
                     ClassOps<LA, LB, LC, LD>::template vrr<double>(
                         F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
-                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.pp.oo2z[pk0 + j], 0.5 * inv, roz, roe, acc);
+                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.pp.oo2z[kj], 0.5 * inv, roz, roe, acc);
                 }
             }
 
@@ -307,18 +289,13 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #undef OUT
         }
 
-        // J on the bra block: converted to fixed point per lane, summed across the
-        // wavefront when every lane shares the bra pair (one atomic per element
-        // instead of up to 64), else added lane by lane.
-        if (bra_uniform) {
+        // J on the bra block: converted to fixed point per lane and summed across
+        // the wavefront (all lanes share the bra pair): one atomic per element
+        // instead of up to 64.
 #pragma unroll
-            for (int k = 0; k < NA * NB; ++k) {
-                const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
-                if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
-            }
-        } else if (active) {
-#pragma unroll
-            for (int k = 0; k < NA * NB; ++k) fx_add(a.X, (long long)(fa + k / NB) * n + fb + k % NB, jab[k]);
+        for (int k = 0; k < NA * NB; ++k) {
+            const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
+            if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
         }
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);

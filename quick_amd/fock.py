@@ -71,6 +71,8 @@ def lib() -> C.CDLL:
         L.qk_num_classes.restype = C.c_int
         L.qk_class_info.argtypes = [P, C.c_int, ip, C.POINTER(C.c_longlong)]
         L.qk_set_timing.argtypes = [P, C.c_int]
+        L.qk_class_lane_fill.restype = C.c_double
+        L.qk_class_lane_fill.argtypes = [P, C.c_int]
         L.qk_class_stats.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double), C.POINTER(C.c_float)]
         _lib = L
@@ -182,7 +184,8 @@ class FockEngine:
             _check(L.qk_class_info(self.h, k, l4, C.byref(w)), "qk_class_info")
             _check(L.qk_class_stats(self.h, k, C.byref(nq), C.byref(npq), C.byref(fl), C.byref(ms)),
                    "qk_class_stats")
-            out.append(("".join("spd"[x] for x in l4), nq.value, npq.value, fl.value, ms.value))
+            out.append(("".join("spd"[x] for x in l4), nq.value, npq.value, fl.value, ms.value,
+                        L.qk_class_lane_fill(self.h, k)))
         return out
 
     def class_work(self):
