@@ -33,6 +33,7 @@ OBJ = HERE / "_build"
 def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ / (src.stem + ".o")
     deps = [src, CSRC / "qk_kernels.h", CSRC / "qk_internal.h", CSRC / "eri_generated.inc",
+            CSRC / "qk_team.h", CSRC / "team_tables.inc",
             HERE.parent / "include" / "quick_mi355x.h", Path(__file__)]
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
         return obj

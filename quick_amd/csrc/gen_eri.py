@@ -264,6 +264,12 @@ def write_units(outdir):
                 f'    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>)) != hipSuccess) return 0;',
                 '    return at.localSizeBytes;',
                 '}']
+        if cls in TEAM_CLASSES:
+            body = body[:2] + ['#include "../team_tables.inc"', '#include "../qk_team.h"'] + body[2:]
+            body += [f'void launch_team_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
+                     f'    hipLaunchKernelGGL((team_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}>), dim3(grid), dim3(64),',
+                     f'                       (team_lds_bytes<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}>()), s, a, w);',
+                     '}']
         if cls[:2] == cls[2:]:
             pt = t[:2]
             body += [f'void launch_schwarz_{pt}(const int* pairs, int np, const int* qa, const int* qb, const int* prim0,',
@@ -318,7 +324,174 @@ def main(path):
         print(tag(cls), nacc, vflop, hflop, nout)
 
 
+# ---------------------------------------------------------------------------
+# Wavefront-cooperative ("team") form of the heavy classes: one wavefront
+# evaluates one quartet, with every intermediate in LDS.  The VRR becomes a
+# node program executed level by level (all nodes of a level are independent),
+# the HRR and the digestion become index tables the lanes stride through.
+
+TEAM_CLASSES = [(2, 1, 2, 1), (2, 2, 1, 1), (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+
+
+def team_program(cls):
+    """Nodes of the VRR DAG for class cls, levelled.  Slot 0 holds 0.0."""
+    la, lb, lc, ld = cls
+    E, F = la + lb, lc + ld
+    targets = [(e, f, 0) for e in erange(la, E) for f in erange(lc, F)]
+    deps_of = {}
+    order = []
+    seen = set()
+
+    def deps(node):
+        e, f, m = node
+        d = {}
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            d["a"], d["b"] = (e, f1, m), (e, f1, m + 1)
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                d["c"], d["d"] = (e, f2, m), (e, f2, m + 1)
+                d["n1"] = f1[i]
+            if e[i] > 0:
+                d["x"] = (sub(e, i), f1, m + 1)
+                d["ne"] = e[i]
+            d["kind"], d["dir"] = 2, i
+        elif sum(e) > 0:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            d["a"], d["b"] = (e1, f, m), (e1, f, m + 1)
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                d["c"], d["d"] = (e2, f, m), (e2, f, m + 1)
+                d["n1"] = e1[i]
+            d["kind"], d["dir"] = 1, i
+        else:
+            d["kind"] = 0
+        return d
+
+    def visit(node):
+        if node in seen:
+            return
+        d = deps(node)
+        deps_of[node] = d
+        for k in ("a", "b", "c", "d", "x"):
+            if k in d:
+                visit(d[k])
+        seen.add(node)
+        order.append(node)
+
+    for t in targets:
+        visit(t)
+    level = {}
+    for node in order:
+        d = deps_of[node]
+        lv = 0
+        for k in ("a", "b", "c", "d", "x"):
+            if k in d:
+                lv = max(lv, level[d[k]] + 1)
+        level[node] = lv
+    nlev = max(level.values()) + 1
+    # slots: 0 = zero, then nodes ordered by level
+    by_level = [[n for n in order if level[n] == L] for L in range(nlev)]
+    slot = {}
+    s = 1
+    for L in range(nlev):
+        for n in by_level[L]:
+            slot[n] = s
+            s += 1
+    nslots = s
+    nodes = []      # uint4 per node
+    lev_start = [0]
+    for L in range(nlev):
+        for n in by_level[L]:
+            d = deps_of[n]
+            g = lambda k: slot[d[k]] if k in d else 0
+            if d["kind"] == 0:
+                flags = (n[2] << 8)            # F_m: m in bits 8..15
+            else:
+                flags = d["kind"] | (d["dir"] << 2) | (d.get("n1", 0) << 4) | (d.get("ne", 0) << 8)
+            nodes.append((slot[n] | (g("a") << 16), g("b") | (g("c") << 16), g("d") | (g("x") << 16), flags))
+        lev_start.append(len(nodes))
+    tslots = [slot[t] for t in targets]
+    return nodes, lev_start, tslots, nslots
+
+
+def binom_terms(a, b):
+    """(a b| = sum_k prod_i C(b_i,k_i) AB_i^(b_i-k_i) (a+k 0|, as (e, px, py, pz, coef)."""
+    from math import comb
+    out = []
+    for kx in range(b[0] + 1):
+        for ky in range(b[1] + 1):
+            for kz in range(b[2] + 1):
+                e = (a[0] + kx, a[1] + ky, a[2] + kz)
+                out.append((e, b[0] - kx, b[1] - ky, b[2] - kz, comb(b[0], kx) * comb(b[1], ky) * comb(b[2], kz)))
+    return out
+
+
+def team_tables(cls):
+    la, lb, lc, ld = cls
+    E, F = la + lb, lc + ld
+    nodes, lev, tslots, nslots = team_program(cls)
+    elist, flist = erange(la, E), erange(lc, F)
+    eidx = {e: k for k, e in enumerate(elist)}
+    fidx = {f: k for k, f in enumerate(flist)}
+    A, B, C, D = QUICK_ORDER[la], QUICK_ORDER[lb], QUICK_ORDER[lc], QUICK_ORDER[ld]
+    bra = []   # per (a,b): 4 terms (eidx, px|py<<2|pz<<4, coef) padded with coef 0
+    for a in A:
+        for b in B:
+            t = binom_terms(a, b)
+            t = [(eidx[e], px | (py << 2) | (pz << 4), c) for e, px, py, pz, c in t]
+            t += [(0, 0, 0)] * (4 - len(t))
+            bra.append(t)
+    ket = []
+    for c in C:
+        for d in D:
+            t = binom_terms(c, d)
+            t = [(fidx[e], px | (py << 2) | (pz << 4), cc) for e, px, py, pz, cc in t]
+            t += [(0, 0, 0)] * (4 - len(t))
+            ket.append(t)
+    return nodes, lev, tslots, nslots, bra, ket, len(elist), len(flist)
+
+
+def write_team(path):
+    import os
+    out = ["// Generated by quick_amd/csrc/gen_eri.py -- do not edit.",
+           "// Node programs and HRR tables of the wavefront-cooperative heavy classes.",
+           "#pragma once", "#include <cstdint>", "namespace qk {"]
+    for cls in TEAM_CLASSES:
+        t = tag(cls)
+        nodes, lev, tslots, nslots, bra, ket, ne, nf = team_tables(cls)
+        out.append(f"__device__ const uint4 kTeamNodes_{t}[{len(nodes)}] = {{")
+        out.append(",\n".join("    {%du, %du, %du, %du}" % nd for nd in nodes))
+        out.append("};")
+        out.append(f"__device__ const int kTeamLevels_{t}[{len(lev)}] = {{{', '.join(map(str, lev))}}};")
+        out.append(f"__device__ const unsigned short kTeamTargets_{t}[{len(tslots)}] = {{{', '.join(map(str, tslots))}}};")
+        flat = lambda tab: ", ".join("%d, %d, %d" % term for row in tab for term in row)
+        out.append(f"__device__ const int kTeamBra_{t}[{len(bra) * 12}] = {{{flat(bra)}}};")
+        out.append(f"__device__ const int kTeamKet_{t}[{len(ket) * 12}] = {{{flat(ket)}}};")
+        out.append(f"struct Team_{t} {{")
+        out.append(f"    static constexpr int kNodes = {len(nodes)}, kLevels = {len(lev) - 1}, kSlots = {nslots};")
+        out.append(f"    static constexpr int kNE = {ne}, kNF = {nf};")
+        out.append(f"    static __device__ __forceinline__ const uint4* nodes() {{ return kTeamNodes_{t}; }}")
+        out.append(f"    static __device__ __forceinline__ const int* levels() {{ return kTeamLevels_{t}; }}")
+        out.append(f"    static __device__ __forceinline__ const unsigned short* targets() {{ return kTeamTargets_{t}; }}")
+        out.append(f"    static __device__ __forceinline__ const int* bra() {{ return kTeamBra_{t}; }}")
+        out.append(f"    static __device__ __forceinline__ const int* ket() {{ return kTeamKet_{t}; }}")
+        out.append("};")
+    out.append("}  // namespace qk")
+    new = "\n".join(out) + "\n"
+    if not os.path.exists(path) or open(path).read() != new:
+        open(path, "w").write(new)
+    for cls in TEAM_CLASSES:
+        nodes, lev, tslots, nslots, bra, ket, ne, nf = team_tables(cls)
+        print(tag(cls), "nodes", len(nodes), "levels", len(lev) - 1, "slots", nslots,
+              "max level", max(lev[i + 1] - lev[i] for i in range(len(lev) - 1)))
+
+
 if __name__ == "__main__":
     main(sys.argv[1])
     if len(sys.argv) > 2:
+        import os
+        write_team(os.path.join(os.path.dirname(sys.argv[1]), "team_tables.inc"))
         write_units(sys.argv[2])

@@ -90,6 +90,19 @@ static const FockLauncher kFockLaunchers[kNumQuartetClasses] = {
 static const SchwarzLauncher kSchwarzLaunchers[kNumPairClasses] = {
     launch_schwarz_ss, launch_schwarz_ps, launch_schwarz_pp, launch_schwarz_ds, launch_schwarz_dp, launch_schwarz_dd};
 
+#define QK_DECL(T) void launch_team_##T(const FockArgs&, const ClassWork&, int, hipStream_t);
+QK_DECL(dpdp) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
+#undef QK_DECL
+// classes evaluated by the wavefront-cooperative kernel (one quartet per wavefront)
+static const FockLauncher kTeamLaunchers[kNumQuartetClasses] = {
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+    nullptr, nullptr, launch_team_dpdp, nullptr, nullptr, launch_team_ddpp, launch_team_ddds, launch_team_dddp,
+    launch_team_dddd};
+bool fock_class_is_team(int k) { return kTeamLaunchers[k] != nullptr; }
+void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+    kTeamLaunchers[k](a, w, grid, s);
+}
+
 using ScratchQuery = size_t (*)();
 static const ScratchQuery kFockScratch[kNumQuartetClasses] = {
     fock_scratch_ssss, fock_scratch_psss, fock_scratch_psps, fock_scratch_ppss, fock_scratch_ppps, fock_scratch_pppp,

@@ -21,6 +21,8 @@
 namespace qk {
 void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 size_t fock_class_scratch(int k);
+bool fock_class_is_team(int k);
+void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_scale_density(const double* D, const double* cons, double* Dp, int n, hipStream_t s);
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
@@ -470,8 +472,11 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                         if (yp * Yp[G2.start + mid] > o->tau) lo = mid + 1; else hi = mid;
                     }
                     const int q_begin = same ? i : 0;
-                    for (int q = q_begin; q < lo; q += 64)
-                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(64, lo - q), 0});
+                    // a wavefront of the cooperative kernel works through its tile's
+                    // quartets one after another: short tiles spread them over the GPU
+                    const int chunk = fock_class_is_team(k) ? 2 : 64;
+                    for (int q = q_begin; q < lo; q += chunk)
+                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
                     if (lo > q_begin) total += lo - q_begin;
                 }
             }
@@ -563,6 +568,17 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         ClassHost& CH = c->classes[k];
         if (CH.tiles.empty()) continue;
         ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size()};
+        a.counters = c->d_counters.p + 2 * k;
+        if (fock_class_is_team(k)) {
+            // one 64-lane block per tile; up to 16 resident per CU
+            const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
+            launch_team_class(k, a, w, grid, s);
+            HIPCHK(hipGetLastError());
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
+            c->cls_timed[k] = c->timing;
+            continue;
+        }
         const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
         long long cap = 256LL * 16;
         const size_t scratch = fock_class_scratch(k);
