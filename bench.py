@@ -36,6 +36,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector (and matrix) peak, AMD spec
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak, AMD spec
 METRIC = "Fock builds/s + gradient s, 32-H2O HF/6-31G**: mixed vs FP64 at ≤1µEh"
 DENSITY_SEED = 7
 
@@ -87,6 +88,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--thresholds", default="1e-3,1e-4,1e-5,1e-6",
+                    help="FP32 thresholds of the mixed-precision sweep ('none': FP64 only)")
+    ap.add_argument("--copy-kernel-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (5 times), to "
+                         "calibrate the FETCH_SIZE/WRITE_SIZE counters, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -94,8 +100,17 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_kernel_mib:
+        n = args.copy_kernel_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_kernel_mib} MiB x 5")
+        return
     from quick_amd.basis import build_basis
-    from quick_amd.fock import FockEngine, make_opts
+    from quick_amd.fock import FockEngine, make_opts, QK_FP64, QK_MIXED
     from quick_amd.geometry import CONFIGS, synthetic_density
 
     syms, xyz = CONFIGS["water32"]()
@@ -104,8 +119,6 @@ def main():
     eng = FockEngine(b)
     t0 = time.time()
     eng.compute_schwarz()
-    opts = make_opts()
-    eng.prepare(opts)
     log(f"system: {len(syms)} atoms, {b.nbasis} bf; setup {time.time() - t0:.3f} s")
 
     dev = torch.device("cuda:0")
@@ -114,46 +127,84 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sptr = stream.cuda_stream
 
-    for _ in range(args.warmup):
-        eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
-    torch.cuda.synchronize()
+    def timed(opts, label):
+        """warmup + `steps` timed builds with `opts`; returns ms/build, G, per-kernel rows"""
+        eng.prepare(opts)
+        for _ in range(args.warmup):
+            eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
+        torch.cuda.synchronize()
+        eng.set_timing(True)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        log(f"{label}: {args.steps} builds {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build")
+        # kernel rows: (kernel, tag, quartets, prim quartets, flops, ms, precision)
+        rows = [(f"fock_kernel<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp64") for c in eng.class_stats()]
+        rows += [(f"fock_kernel_f32<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp32")
+                 for c in eng.class_stats_fp32()]
+        rows = [r for r in rows if r[2]]
+        G = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
+        return ms, G, rows, eng.stats()
 
-    eng.set_timing(True)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
-    log(f"{args.steps} builds: {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build")
+    def show(rows):
+        for r in sorted(rows, key=lambda r: -r[5]):
+            log(f"  {r[0]:<24s} {r[2]:>10d} quartets {r[3]:>11d} prim {r[5]:8.3f} ms "
+                f"{r[4] / max(r[5], 1e-9) / 1e9:8.2f} TFLOP/s")
 
-    cls = eng.class_stats()
-    st = eng.stats()
-    dom = max(cls, key=lambda c: c[4])
-    tag, nq, npq, flops, kms, _ = dom
+    # all-FP64 build (the parity path)
+    ms64, G64, rows64, st64 = timed(make_opts(precision=QK_FP64), "fp64")
+    show(rows64)
+    E64 = 0.5 * float(np.sum(D * G64))
+
+    # mixed precision: FP32 below the Schwarz-bound threshold; the threshold is
+    # the largest of the sweep whose energy error against FP64 is <= 1 uEh
+    sweep = []
+    thresholds = [] if args.thresholds == "none" else [float(t) for t in args.thresholds.split(",")]
+    for thr in thresholds:
+        msm, Gm, rowsm, stm = timed(make_opts(precision=QK_MIXED, fp32_threshold=thr), f"mixed {thr:.0e}")
+        dE = abs(0.5 * float(np.sum(D * Gm)) - E64)
+        dF = float(np.abs(Gm - G64).max())
+        sweep.append({"fp32_threshold": thr, "ms_per_build": msm, "builds_per_s": 1000.0 / msm,
+                      "fp32_quartet_share": stm.quartets_fp32 / max(stm.quartets, 1),
+                      "abs_dE_vs_fp64": dE, "max_abs_dF_vs_fp64": dF, "_rows": rowsm, "_st": stm})
+        log(f"  fp32 share {stm.quartets_fp32 / max(stm.quartets, 1):.3f} |dE| {dE:.2e} max|dF| {dF:.2e}")
+    ok = [r for r in sweep if r["abs_dE_vs_fp64"] <= 1e-6]
+    pick = max(ok, key=lambda r: r["builds_per_s"]) if ok else None
+    if pick is not None:
+        ms, rows, st, precision = pick["ms_per_build"], pick["_rows"], pick["_st"], "mixed"
+        show(rows)
+    else:
+        ms, rows, st, precision = ms64, rows64, st64, "fp64"
+    for r in sweep:
+        r.pop("_rows"), r.pop("_st")
+
+    dom = max(rows, key=lambda r: r[5])
+    kname, tag, nq, npq, flops, kms, kprec = dom
+    peak = PEAK_FP64_TFLOPS if kprec == "fp64" else PEAK_FP32_TFLOPS
     achieved = flops / (kms * 1e-3) / 1e12
-    total_flops = sum(c[3] for c in cls)
-    for c in sorted(cls, key=lambda c: -c[4]):
-        if c[1]:
-            log(f"  class {c[0]}: {c[1]:>10d} quartets {c[2]:>11d} prim {c[4]:8.3f} ms "
-                f"{c[3] / max(c[4], 1e-9) / 1e9:8.2f} TFLOP/s  lane fill {c[5]:.2f}")
-    Gh = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
-    e2 = 0.5 * float(np.sum(D * Gh))
+    total64 = sum(r[4] for r in rows if r[6] == "fp64")
+    total32 = sum(r[4] for r in rows if r[6] == "fp32")
+    # mixed-mode bound: time at peak for the FP64 and the FP32 flops
+    t_peak_ms = (total64 / (PEAK_FP64_TFLOPS * 1e12) + total32 / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
 
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic_water32.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get(f"fock_{tag}")
+            traffic = json.loads(tfile.read_text()).get(kname)
         except Exception:
             traffic = None
 
     base = None if args.no_cpu_baseline else cpu_baseline(syms, xyz, D)
     value = 1000.0 / ms
+    thr = pick["fp32_threshold"] if pick is not None else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -165,17 +216,24 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if precision == "mixed" else "f64",
         "data": "synthetic: seeded (H2O)32 geometry, seeded exp(-0.25 R^2) density",
         "config": {"workload": "(H2O)32 HF/6-31G** two-electron Fock build G(D), 800 bf, tau 1e-11, "
-                               "all-FP64 ERIs", "atoms": len(syms), "nbasis": int(b.nbasis),
-                   "precision": "fp64", "geometry_seed": 2, "density_seed": DENSITY_SEED,
-                   "quartets": int(st.quartets), "prim_quartets": int(st.prim_quartets),
-                   "E2_half_trDG": e2, "gradient_s": None},
+                               + ("mixed FP32/FP64 ERIs (FP32 below Schwarz bound %g), FP64 accumulation" % thr
+                                  if precision == "mixed" else "all-FP64 ERIs"),
+                   "atoms": len(syms), "nbasis": int(b.nbasis),
+                   "precision": precision, "fp32_threshold": thr, "geometry_seed": 2,
+                   "density_seed": DENSITY_SEED, "quartets": int(st.quartets),
+                   "quartets_fp32": int(st.quartets_fp32), "prim_quartets": int(st.prim_quartets),
+                   "E2_half_trDG_fp64": E64, "gradient_s": None},
+        "fp64": {"builds_per_s": 1000.0 / ms64, "ms_per_build": ms64},
+        "mixed_vs_fp64_speedup": ms64 / ms,
+        "mixed_sweep": sweep,
         "eri_quartets_per_s": st.quartets * value,
-        "achieved_fp64_tflops_total": total_flops / (ms * 1e-3) / 1e12,
-        "roofline": {"bound": "valu-fp64", "kernel": f"fock_kernel<{tag}>", "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+        "achieved_tflops_total": (total64 + total32) / (ms * 1e-3) / 1e12,
+        "vector_roofline_time_frac": t_peak_ms / ms,
+        "roofline": {"bound": "valu-" + kprec, "kernel": kname, "achieved": achieved,
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": kms},
         "cpu_baseline": base,
     }

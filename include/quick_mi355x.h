@@ -133,7 +133,10 @@ int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, vo
 /* Host convenience: F <- F + G(D) (host arrays), synchronous */
 int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 
-/* Backward: dgrad (3*natom, device) <- dgrad + dE_2e/dR for density dD. */
+/* Backward: dgrad (3*natom, device) <- dgrad + dE2/dR at fixed density dD,
+ * E2 = 1/2 sum D G(D): the four-centre term of hfgrad (gradient.f90:261-320,
+ * shell.f90:1863-2535, hrr.f90:862-1014).  opts->tau is the gradient cutoff
+ * (quick_method%gradCutoff); FP64 only.  Runs on `stream`. */
 int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
 int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 
@@ -157,10 +160,19 @@ double qk_class_lane_fill(qk_ctx* c, int k);
 /* Record HIP events around every class kernel of the following builds (on the
  * stream the kernels run on). */
 int qk_set_timing(qk_ctx* c, int on);
-/* Figures of class k in the last build: surviving quartets, primitive
- * quartets, algorithmic flops; and the class kernel's average time in ms over
- * all builds since qk_set_timing(c, 1) (0 unless timing was on). */
+/* Figures of class k's FP64 kernel in the last build: surviving quartets,
+ * primitive quartets, algorithmic flops; and the kernel's average time in ms
+ * over all builds since qk_set_timing(c, 1) (0 unless timing was on). */
 int qk_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
+/* Surviving quartets and primitive quartets of class k in the last gradient. */
+int qk_grad_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets);
+/* The same figures for class k's FP32 kernel (QK_MIXED; zero otherwise). */
+int qk_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
+
+/* The FP32 kernels' Boys table: F_m(k / per_unit) for k < rows, m < cols,
+ * row-major.  No device needed. */
+void qk_boys_table_shape(int* rows, int* cols, int* per_unit);
+int qk_boys_table_f32(float* out, int rows, int cols);
 
 #ifdef __cplusplus
 }

@@ -48,9 +48,8 @@ def lib():
                                C.POINTER(C.c_long)]
         L.qko_eri.restype = C.c_double
         L.qko_eri.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
-        if hasattr(L, "qko_grad"):
-            L.qko_grad.restype = C.c_long
-            L.qko_grad.argtypes = [P, dp, dp, C.c_double, C.c_double, C.c_int]
+        L.qko_grad.restype = C.c_long
+        L.qko_grad.argtypes = [P, dp, dp, C.c_double, C.c_double, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -105,6 +104,14 @@ class Oracle:
         nq = lib().qko_fock(self.h, _f(D, np.float64), G, tau, prim_limit, pair_cut, hybrid,
                             {"cpu": 0, "gpu": 1, "gpu_cpuboys": 2}[mode], C.byref(npq))
         return G.reshape(self.nbasis, self.nbasis, order="F"), int(nq), int(npq.value)
+
+    def grad(self, D, cutoff=1e-11, hybrid=1.0):
+        """Four-centre gradient dE2/dR (3 x natom) at fixed D, E2 = 1/2 sum D G(D),
+        with hfgrad's enumeration and screening (cutoff = gradCutoff)."""
+        g = np.zeros(3 * self.natom)
+        npq = C.c_long(0)
+        nq = lib().qko_grad(self.h, _f(D, np.float64), g, cutoff, hybrid, C.byref(npq))
+        return g.reshape(3, self.natom, order="F"), int(nq), int(npq.value)
 
     def eri(self, i, j, k, l):
         return lib().qko_eri(self.h, i, j, k, l)

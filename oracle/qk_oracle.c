@@ -707,6 +707,211 @@ long qko_fock(qko_t *c, const double *D, double *G, double tau, double primLimit
     return nq;
 }
 
+/* ---------------- four-centre gradient ---------------- */
+/* Contracted stores for the first derivative (shellopt + classopt,
+ * shell.f90:1863-2114): VRR raised by one on both sides, FmT asked for order
+ * Mboys, primitive quartets kept when dnmax*cutprim*cutprim > cutoff.
+ * gstore[0] carries the plain contraction weights X44, gstore[1..3] the same
+ * times 2a, 2b, 2c (exponents of the primitives on A, B, C). */
+static double gstore[4][NCART][NCART];
+
+static long contracted_store_grad(qko_t *c, int II, int JJ, int KK, int LL, int I, int J, int K, int L,
+                                  double dnmax, double cutoff, int Mboys) {
+    int jb = c->jbasis;
+    int LE = I + J + 1, LF = K + L + 1;
+    int ne = ncart_upto[LE + 1], nf = ncart_upto[LF + 1];
+    for (int g = 0; g < 4; ++g)
+        for (int e = 0; e < ne; ++e) for (int f = 0; f < nf; ++f) gstore[g][e][f] = 0.0;
+    const double X0 = 2.0 * pow(PI, 2.5);
+    double RA[3], RC[3];
+    for (int m = 0; m < 3; ++m) { RA[m] = ATOMXYZ(m, c->katom[II] - 1); RC[m] = ATOMXYZ(m, c->katom[KK] - 1); }
+    long nq = 0;
+    double Fm[MMAX + 1];
+    for (int jj = 0; jj < c->kprim[JJ]; ++jj)
+        for (int ii = 0; ii < c->kprim[II]; ++ii) {
+            int Pi = c->kstart[II] - 1 + ii, Pj = c->kstart[JJ] - 1 + jj;
+            double AB = c->Apri[IX2(Pi, Pj, jb)];
+            const double *P = &c->Ppri[3 * IX2(Pi, Pj, jb)];
+            double X2 = X0 * XC(Pi, Pj, I, J);
+            double cp1 = dnmax * c->cutprim[IX2(Pi, Pj, jb)];
+            double aa = GCE(ii, c->ksumtype[II] - 1), bb = GCE(jj, c->ksumtype[JJ] - 1);
+            for (int ll = 0; ll < c->kprim[LL]; ++ll)
+                for (int kk = 0; kk < c->kprim[KK]; ++kk) {
+                    int Pk = c->kstart[KK] - 1 + kk, Pl = c->kstart[LL] - 1 + ll;
+                    if (!(cp1 * c->cutprim[IX2(Pk, Pl, jb)] > cutoff)) continue;
+                    double cc = GCE(kk, c->ksumtype[KK] - 1);
+                    double CD = c->Apri[IX2(Pk, Pl, jb)];
+                    const double *Q = &c->Ppri[3 * IX2(Pk, Pl, jb)];
+                    double ABCD = AB + CD, ROU = AB * CD / ABCD, RPQ = 0.0;
+                    vrr_par vp;
+                    for (int m = 0; m < 3; ++m) {
+                        double W = (P[m] * AB + Q[m] * CD) / ABCD;
+                        double d = P[m] - Q[m];
+                        RPQ += d * d;
+                        vp.PA[m] = P[m] - RA[m];
+                        vp.QC[m] = Q[m] - RC[m];
+                        vp.WP[m] = W - P[m];
+                        vp.WQ[m] = W - Q[m];
+                    }
+                    vp.oo2z = 0.5 / AB; vp.oo2e = 0.5 / CD; vp.oo2ze = 0.5 / ABCD;
+                    vp.roz = CD / ABCD; vp.roe = AB / ABCD;
+                    double T = RPQ * ROU;
+                    if (c->boys_exact) qko_fmt_exact(Mboys, T, Fm); else qko_fmt(Mboys, T, Fm);
+                    double s = 1.0 / sqrt(ABCD);
+                    for (int m = 0; m <= Mboys; ++m) Fm[m] *= s;
+                    vrr(LE, LF, Fm, &vp, vtmp);
+                    double X44 = X2 * XC(Pk, Pl, K, L);
+                    double w[4] = {X44, X44 * aa * 2.0, X44 * bb * 2.0, X44 * cc * 2.0};
+                    for (int g = 0; g < 4; ++g)
+                        for (int e = 0; e < ne; ++e)
+                            for (int f = 0; f < nf; ++f) gstore[g][e][f] += w[g] * vtmp[e][f];
+                    ++nq;
+                }
+        }
+    return nq;
+}
+
+/* sum over the HRR expansions of (a b| and |c d) of store[e][f] */
+static double hrr_pair(const int *a, const int *b, const int *cc, const int *d, const double *AB, const double *CD,
+                       double st[NCART][NCART]) {
+    double cl_[32], cr[32];
+    int il[32], ir[32];
+    int nl = hrr_expand(a, b, AB, cl_, il);
+    int nr = hrr_expand(cc, d, CD, cr, ir);
+    double Y = 0.0;
+    for (int x = 0; x < nl; ++x)
+        for (int y = 0; y < nr; ++y) Y += cl_[x] * cr[y] * st[il[x]][ir[y]];
+    return Y;
+}
+
+/* d(iii jjj|kkk lll)/dA, /dB, /dC (hrrwholeopt, subs/hrr.f90:862-1014):
+ * d/dA_x (a b| = 2a (a+1_x b| - a_x (a-1_x b|, likewise for B and C, with the
+ * exponent factors carried by gstore[1..3]; times the four cons factors. */
+static void hrr_grad(qko_t *c, int II, int JJ, int KK, int LL, int iii, int jjj, int kkk, int lll,
+                     double *Yaa, double *Ybb, double *Ycc) {
+    double AB[3], CD[3];
+    for (int m = 0; m < 3; ++m) {
+        AB[m] = ATOMXYZ(m, c->katom[II] - 1) - ATOMXYZ(m, c->katom[JJ] - 1);
+        CD[m] = ATOMXYZ(m, c->katom[KK] - 1) - ATOMXYZ(m, c->katom[LL] - 1);
+    }
+    const int *a = &c->KLMN[3 * iii], *b = &c->KLMN[3 * jjj], *cc = &c->KLMN[3 * kkk], *d = &c->KLMN[3 * lll];
+    double cons4 = c->cons[iii] * c->cons[jjj] * c->cons[kkk] * c->cons[lll];
+    for (int x = 0; x < 3; ++x) {
+        int t[3];
+        memcpy(t, a, sizeof t); t[x] += 1;
+        double v = hrr_pair(t, b, cc, d, AB, CD, gstore[1]);
+        if (a[x] >= 1) { memcpy(t, a, sizeof t); t[x] -= 1; v -= a[x] * hrr_pair(t, b, cc, d, AB, CD, gstore[0]); }
+        Yaa[x] = v * cons4;
+        memcpy(t, b, sizeof t); t[x] += 1;
+        v = hrr_pair(a, t, cc, d, AB, CD, gstore[2]);
+        if (b[x] >= 1) { memcpy(t, b, sizeof t); t[x] -= 1; v -= b[x] * hrr_pair(a, t, cc, d, AB, CD, gstore[0]); }
+        Ybb[x] = v * cons4;
+        memcpy(t, cc, sizeof t); t[x] += 1;
+        v = hrr_pair(a, b, t, d, AB, CD, gstore[3]);
+        if (cc[x] >= 1) { memcpy(t, cc, sizeof t); t[x] -= 1; v -= cc[x] * hrr_pair(a, b, t, d, AB, CD, gstore[0]); }
+        Ycc[x] = v * cons4;
+    }
+}
+
+/* The density factor of one unique function quartet in classopt's branches
+ * (shell.f90:2190-2460), with the exchange part scaled by hyb (1 = HF, as the
+ * reference). Returns 0 for quartets the branches skip. */
+static double grad_constant(const double *D, int n, int III, int JJJ, int KKK, int LLL, double hyb, int *use) {
+    *use = 1;
+    if (III < KKK) {
+        if (III < JJJ && KKK < LLL)
+            return 4.0 * DEN(JJJ, III) * DEN(LLL, KKK) - hyb * (DEN(KKK, III) * DEN(LLL, JJJ) + DEN(LLL, III) * DEN(KKK, JJJ));
+        if (III == JJJ && KKK == LLL)
+            return DEN(III, III) * DEN(KKK, KKK) - hyb * 0.5 * DEN(KKK, III) * DEN(KKK, III);
+        if (JJJ == KKK && JJJ == LLL)
+            return (2.0 - hyb) * DEN(JJJ, JJJ) * DEN(JJJ, III);
+        if (KKK == LLL && III < JJJ && JJJ != KKK)
+            return 2.0 * DEN(JJJ, III) * DEN(KKK, KKK) - hyb * DEN(KKK, III) * DEN(KKK, JJJ);
+        if (III == JJJ && KKK < LLL)
+            return 2.0 * DEN(LLL, KKK) * DEN(III, III) - hyb * DEN(KKK, III) * DEN(LLL, III);
+        *use = 0;
+        return 0.0;
+    }
+    if (JJJ <= LLL) {
+        if (III == JJJ && III == KKK && III == LLL) { *use = 0; return 0.0; }
+        if (III == JJJ && III == KKK && III < LLL) return (2.0 - hyb) * DEN(LLL, III) * DEN(III, III);
+        if (III == KKK && JJJ == LLL && III < JJJ)
+            return (2.0 - 0.5 * hyb) * DEN(JJJ, III) * DEN(JJJ, III) - 0.5 * hyb * DEN(JJJ, JJJ) * DEN(III, III);
+        if (III == KKK && III < JJJ && JJJ < LLL)
+            return (4.0 - hyb) * DEN(JJJ, III) * DEN(LLL, III) - hyb * DEN(LLL, JJJ) * DEN(III, III);
+    }
+    *use = 0;
+    return 0.0;
+}
+
+/* Four-centre part of the nuclear gradient, dE2/dR with E2 = 1/2 sum D G(D) at
+ * fixed D (hfgrad, gradient.f90:266-320): shell quartets II<=JJ, II<=KK<=LL,
+ * one-centre quartets skipped, Y_IJ*Y_KL > cutoff and Y_IJ*Y_KL*DNmax > cutoff,
+ * FmT order = sum of the four shells' Qfinal + 2 (shell.f90:1906-1912);
+ * dE/dD = -(dE/dA + dE/dB + dE/dC) (classopt:2463-2520).
+ * grad is 3 x natom (column-major), accumulated into. */
+long qko_grad(qko_t *c, const double *D, double *grad, double cutoff, double hyb, long *nprimq) {
+    int ns = c->nshell, n = c->nbasis;
+    double *cm = calloc((size_t)ns * ns, sizeof(double));
+    density_cutoff(c, D, cm);
+    const double *Y = c->Ycutoff;
+    long nq = 0, np = 0;
+    for (int II = 0; II < ns; ++II)
+        for (int JJ = II; JJ < ns; ++JJ)
+            for (int KK = II; KK < ns; ++KK)
+                for (int LL = KK; LL < ns; ++LL) {
+                    int a0 = c->katom[II];
+                    if (a0 == c->katom[JJ] && a0 == c->katom[KK] && a0 == c->katom[LL]) continue;
+                    double ct = Y[IX2(II, JJ, ns)] * Y[IX2(KK, LL, ns)];
+                    if (!(ct > cutoff)) continue;
+                    double dn = dnmax_of(cm, ns, II, JJ, KK, LL);
+                    if (!(ct * dn > cutoff)) continue;
+                    ++nq;
+                    int Mb = c->Qfinal[II] + c->Qfinal[JJ] + c->Qfinal[KK] + c->Qfinal[LL] + 2;
+                    int iA = c->katom[II] - 1, iB = c->katom[JJ] - 1, iC = c->katom[KK] - 1, iD = c->katom[LL] - 1;
+                    for (int I = c->Qstart[II]; I <= c->Qfinal[II]; ++I)
+                        for (int J = c->Qstart[JJ]; J <= c->Qfinal[JJ]; ++J)
+                            for (int K = c->Qstart[KK]; K <= c->Qfinal[KK]; ++K)
+                                for (int L = c->Qstart[LL]; L <= c->Qfinal[LL]; ++L) {
+                                    np += contracted_store_grad(c, II, JJ, KK, LL, I, J, K, L, dn, cutoff, Mb);
+                                    int III1, III2, JJJ1, JJJ2, KKK1, KKK2, LLL1, LLL2;
+                                    qrange(c, II, I, &III1, &III2); qrange(c, JJ, J, &JJJ1, &JJJ2);
+                                    qrange(c, KK, K, &KKK1, &KKK2); qrange(c, LL, L, &LLL1, &LLL2);
+                                    double Ag[3] = {0, 0, 0}, Bg[3] = {0, 0, 0}, Cg[3] = {0, 0, 0};
+                                    double Yaa[3], Ybb[3], Ycc[3];
+                                    int fast = II < JJ && II < KK && KK < LL;
+                                    for (int III = III1; III <= III2; ++III)
+                                        for (int JJJ = fast ? JJJ1 : (III > JJJ1 ? III : JJJ1); JJJ <= JJJ2; ++JJJ)
+                                            for (int KKK = fast ? KKK1 : (III > KKK1 ? III : KKK1); KKK <= KKK2; ++KKK)
+                                                for (int LLL = fast ? LLL1 : (KKK > LLL1 ? KKK : LLL1); LLL <= LLL2; ++LLL) {
+                                                    int use = 1;
+                                                    double k;
+                                                    if (fast)
+                                                        k = 4.0 * DEN(JJJ, III) * DEN(LLL, KKK) -
+                                                            hyb * (DEN(KKK, III) * DEN(LLL, JJJ) + DEN(LLL, III) * DEN(KKK, JJJ));
+                                                    else
+                                                        k = grad_constant(D, n, III, JJJ, KKK, LLL, hyb, &use);
+                                                    if (!use) continue;
+                                                    hrr_grad(c, II, JJ, KK, LL, III, JJJ, KKK, LLL, Yaa, Ybb, Ycc);
+                                                    for (int x = 0; x < 3; ++x) {
+                                                        Ag[x] += Yaa[x] * k;
+                                                        Bg[x] += Ybb[x] * k;
+                                                        Cg[x] += Ycc[x] * k;
+                                                    }
+                                                }
+                                    for (int x = 0; x < 3; ++x) {
+                                        grad[3 * iA + x] += Ag[x];
+                                        grad[3 * iB + x] += Bg[x];
+                                        grad[3 * iC + x] += Cg[x];
+                                        grad[3 * iD + x] += -Ag[x] - Bg[x] - Cg[x];
+                                    }
+                                }
+                }
+    free(cm);
+    if (nprimq) *nprimq = np;
+    return nq;
+}
+
 /* Contracted (ij|kl) for basis functions (0-based), no screening. */
 double qko_eri(qko_t *c, int i, int j, int k, int l) {
     int sh[4], ty[4], f[4] = {i, j, k, l};

@@ -2,9 +2,9 @@
 
     python -m quick_amd.build        (or __graft_entry__.build())
 
-Steps: regenerate the per-class VRR/HRR device code (csrc/gen_eri.py), then one
-hipcc invocation compiles the kernels and the host/C-ABI code into a shared
-library next to this file.
+Steps: regenerate the per-class VRR/HRR device code (csrc/gen_eri.py) and the
+gradient tables (csrc/gen_grad.py), compile every translation unit with hipcc
+(in parallel, into _build/), and link the shared library next to this file.
 """
 from __future__ import annotations
 
@@ -33,7 +33,7 @@ OBJ = HERE / "_build"
 def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ / (src.stem + ".o")
     deps = [src, CSRC / "qk_kernels.h", CSRC / "qk_internal.h", CSRC / "eri_generated.inc",
-            CSRC / "qk_team.h", CSRC / "team_tables.inc",
+            CSRC / "qk_team.h", CSRC / "team_tables.inc", CSRC / "grad_tables.inc",
             HERE.parent / "include" / "quick_mi355x.h", Path(__file__)]
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
         return obj
@@ -50,8 +50,10 @@ def build(verbose: bool = True, jobs: int = 8) -> Path:
     OBJ.mkdir(exist_ok=True)
     subprocess.run([sys.executable, str(CSRC / "gen_eri.py"), str(CSRC / "eri_generated.inc"),
                     str(CSRC / "gen")], check=True, stdout=subprocess.DEVNULL)
+    subprocess.run([sys.executable, str(CSRC / "gen_grad.py"), str(CSRC / "grad_tables.inc")], check=True,
+                   stdout=subprocess.DEVNULL)
     srcs = sorted((CSRC / "gen").glob("fock_*.hip"), key=lambda p: -p.stat().st_size)
-    srcs += [CSRC / "qk_fock.hip", CSRC / "qk_host.cpp"]
+    srcs += [CSRC / "qk_fock.hip", CSRC / "qk_grad.hip", CSRC / "qk_host.cpp"]
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -264,6 +264,15 @@ def write_units(outdir):
                 f'    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>)) != hipSuccess) return 0;',
                 '    return at.localSizeBytes;',
                 '}']
+        if cls not in TEAM_CLASSES:
+            body += [f'void launch_fock_f32_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
+                     f'    hipLaunchKernelGGL((fock_kernel_f32<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), 0, s, a, w);',
+                     '}',
+                     f'size_t fock_scratch_f32_{t}() {{',
+                     '    hipFuncAttributes at{};',
+                     f'    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(&fock_kernel_f32<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>)) != hipSuccess) return 0;',
+                     '    return at.localSizeBytes;',
+                     '}']
         if cls in TEAM_CLASSES:
             body = body[:2] + ['#include "../team_tables.inc"', '#include "../qk_team.h"'] + body[2:]
             body += [f'void launch_team_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',

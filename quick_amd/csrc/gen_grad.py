@@ -1,0 +1,242 @@
+"""Generator of the gradient kernel's per-class tables (grad_tables.inc).
+
+The nuclear-gradient kernel (qk_grad.hip) is one table-driven wavefront-
+cooperative kernel for every quartet class.  For class (la lb|lc ld) it needs:
+
+* a VRR node program (the Obara-Saika DAG, levelled; the same node encoding as
+  the Fock team kernel) whose targets are the [e0|f0] of four contracted sets,
+  the VRR raised by one on each side (shell.f90:1906-1912, 1968):
+      S0: |e| in [la-1, la+lb],   |f| in [lc-1, lc+ld]   weight 1
+      SA: |e| in [la+1, la+lb+1], |f| in [lc, lc+ld]     weight 2 alpha
+      SB: |e| in [la, la+lb+1],   |f| in [lc, lc+ld]     weight 2 beta
+      SC: |e| in [la, la+lb],     |f| in [lc+1, lc+ld+1] weight 2 gamma
+  (alpha, beta, gamma: exponents of the primitives on A, B, C; classopt,
+  shell.f90:2056-2114);
+* per derivative term ("stage") a bra HRR table, a ket HRR table and a
+  contraction table.  d/dA_x (ab|cd) = 2 alpha (a+1x b|cd) - a_x (a-1x b|cd)
+  and likewise for B and C (hrrwholeopt, subs/hrr.f90:862-1014); the stages
+  are A+ (from SA), A- (S0), B+ (SB), B- (S0), C+ (SC), C- (S0).
+
+Everything is emitted as one int32 blob per class with a header of offsets
+(layout in qk_internal.h, GradHdr) that the host uploads once.
+"""
+import os
+import sys
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from gen_eri import QUICK_ORDER, carts, erange, first_dir, sub, tag, quartet_classes, binom_terms  # noqa: E402
+
+MAX_TERMS = 8       # HRR terms per output row: prod (b_i + 1) <= 8 for |b| <= 3
+N_STAGE_FIELDS = 8  # dir, src, nR1, nCol, nR2, off_bra, off_ket, off_con
+HDR = 24            # fixed header fields before the stage records
+
+
+def order(L):
+    """Cartesian order of shell L: QUICK's function order for s/p/d, carts(3) for f."""
+    return QUICK_ORDER[L] if L in QUICK_ORDER else carts(L)
+
+
+def grad_program(sets):
+    """Levelled VRR node program for the union of the sets' targets."""
+    targets = []
+    for E, F in sets:
+        targets += [(e, f, 0) for e in E for f in F]
+    deps_of, order_, seen = {}, [], set()
+
+    def deps(node):
+        e, f, m = node
+        d = {}
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            d["a"], d["b"] = (e, f1, m), (e, f1, m + 1)
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                d["c"], d["d"] = (e, f2, m), (e, f2, m + 1)
+                d["n1"] = f1[i]
+            if e[i] > 0:
+                d["x"] = (sub(e, i), f1, m + 1)
+                d["ne"] = e[i]
+            d["kind"], d["dir"] = 2, i
+        elif sum(e) > 0:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            d["a"], d["b"] = (e1, f, m), (e1, f, m + 1)
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                d["c"], d["d"] = (e2, f, m), (e2, f, m + 1)
+                d["n1"] = e1[i]
+            d["kind"], d["dir"] = 1, i
+        else:
+            d["kind"] = 0
+        return d
+
+    def visit(node):
+        if node in seen:
+            return
+        d = deps(node)
+        deps_of[node] = d
+        for k in ("a", "b", "c", "d", "x"):
+            if k in d:
+                visit(d[k])
+        seen.add(node)
+        order_.append(node)
+
+    for t in targets:
+        visit(t)
+    level = {}
+    for node in order_:
+        d = deps_of[node]
+        lv = 0
+        for k in ("a", "b", "c", "d", "x"):
+            if k in d:
+                lv = max(lv, level[d[k]] + 1)
+        level[node] = lv
+    nlev = max(level.values()) + 1
+    by_level = [[n for n in order_ if level[n] == L] for L in range(nlev)]
+    slot, s = {}, 1
+    for L in range(nlev):
+        for n in by_level[L]:
+            slot[n] = s
+            s += 1
+    nodes, lev_start = [], [0]
+    for L in range(nlev):
+        for n in by_level[L]:
+            d = deps_of[n]
+            g = lambda k: slot[d[k]] if k in d else 0
+            if d["kind"] == 0:
+                flags = n[2] << 8
+            else:
+                flags = d["kind"] | (d["dir"] << 2) | (d.get("n1", 0) << 4) | (d.get("ne", 0) << 8)
+            nodes.append((slot[n] | (g("a") << 16), g("b") | (g("c") << 16), g("d") | (g("x") << 16), flags))
+        lev_start.append(len(nodes))
+    tslots = [[slot[(e, f, 0)] for e in E for f in F] for E, F in sets]
+    return nodes, lev_start, tslots, s
+
+
+def hrr_rows(l1, l2, src_list):
+    """(a b| for a in order(l1), b in order(l2) from rows of src_list: MAX_TERMS x
+    (row, powcode, coef), padded with coef 0."""
+    idx = {e: k for k, e in enumerate(src_list)}
+    rows = []
+    for a in order(l1):
+        for b in order(l2):
+            t = [(idx[e], px | (py << 2) | (pz << 4), c) for e, px, py, pz, c in binom_terms(a, b)]
+            assert len(t) <= MAX_TERMS
+            t += [(0, 0, 0)] * (MAX_TERMS - len(t))
+            rows.append(t)
+    return rows
+
+
+def class_blob(cls):
+    la, lb, lc, ld = cls
+    r = lambda lo, hi: erange(max(lo, 0), hi)
+    E0, F0 = r(la - 1, la + lb), r(lc - 1, lc + ld)
+    EA, FA = r(la + 1, la + lb + 1), r(lc, lc + ld)
+    EB, FB = r(la, la + lb + 1), r(lc, lc + ld)
+    EC, FC = r(la, la + lb), r(lc + 1, lc + ld + 1)
+    sets = [(E0, F0), (EA, FA), (EB, FB), (EC, FC)]
+    nodes, lev, tslots, nslots = grad_program(sets)
+    NA, NB, NC, ND = (len(order(l)) for l in cls)
+    nout = NA * NB * NC * ND
+    # stages: (dir, src set, bra (l1,l2), ket (l3,l4), shifted shell index, sign)
+    stages = [(0, 1, (la + 1, lb), (lc, ld), 0, +1)]
+    if la >= 1:
+        stages.append((0, 0, (la - 1, lb), (lc, ld), 0, -1))
+    stages.append((1, 2, (la, lb + 1), (lc, ld), 1, +1))
+    if lb >= 1:
+        stages.append((1, 0, (la, lb - 1), (lc, ld), 1, -1))
+    stages.append((2, 3, (la, lb), (lc + 1, ld), 2, +1))
+    if lc >= 1:
+        stages.append((2, 0, (la, lb), (lc - 1, ld), 2, -1))
+
+    blob = [0] * (HDR + N_STAGE_FIELDS * 6)
+    body = []
+
+    def put(vals):
+        off = len(blob) + len(body)
+        body.extend(vals)
+        return off
+
+    off_nodes = put([v for nd in nodes for v in nd])
+    off_lev = put(lev)
+    off_tgt = put([s for ts in tslots for s in ts])
+    stage_recs = []
+    base_l = [la, lb, lc, ld]
+    for dirn, src, (l1, l2), (l3, l4), which, sign in stages:
+        Es, Fs = sets[src]
+        bra = hrr_rows(l1, l2, Es)
+        ket = hrr_rows(l3, l4, Fs)
+        nR1, nR2, ncol = len(bra), len(ket), len(Fs)
+        off_bra = put([v for row in bra for term in row for v in term])
+        off_ket = put([v for row in ket for term in row for v in term])
+        # contraction: for each function quartet of the class and axis, the
+        # element of this stage's output and its factor (+1 or -n_x)
+        shp = list(base_l)
+        shp[which] += sign
+        o1 = {a: k for k, a in enumerate(order(shp[0]))}
+        o2 = {b: k for k, b in enumerate(order(shp[1]))}
+        o3 = {c: k for k, c in enumerate(order(shp[2]))}
+        o4 = {d: k for k, d in enumerate(order(shp[3]))}
+        n2, n3, n4 = len(o2), len(o3), len(o4)
+        con = []
+        for a in order(la):
+            for b in order(lb):
+                for c in order(lc):
+                    for d in order(ld):
+                        for x in range(3):
+                            v = [a, b, c, d]
+                            nx = v[which][x]
+                            if sign < 0 and nx == 0:
+                                con += [-1, 0]
+                                continue
+                            w = list(v[which])
+                            w[x] += sign
+                            v[which] = tuple(w)
+                            i1, i2, i3, i4 = o1[v[0]], o2[v[1]], o3[v[2]], o4[v[3]]
+                            con += [(i1 * n2 + i2) * (n3 * n4) + i3 * n4 + i4, 1 if sign > 0 else -nx]
+        off_con = put(con)
+        assert nR1 == len(o1) * n2 and nR2 == n3 * n4
+        stage_recs.append([dirn, src, nR1, ncol, nR2, off_bra, off_ket, off_con])
+    n_sets = [len(E) * len(F) for E, F in sets]
+    max_stage = max(rec[2] * rec[3] + rec[2] * rec[4] for rec in stage_recs)
+    hdr = [len(nodes), len(lev) - 1, nslots, la + lb + lc + ld,
+           n_sets[0], n_sets[1], n_sets[2], n_sets[3],
+           len(F0), len(FA), len(FB), len(FC), len(stage_recs), nout, NA, NB, NC, ND,
+           off_nodes, off_lev, off_tgt, max_stage, 0, 0]
+    assert len(hdr) == HDR
+    blob[:HDR] = hdr
+    for k, rec in enumerate(stage_recs):
+        blob[HDR + N_STAGE_FIELDS * k: HDR + N_STAGE_FIELDS * (k + 1)] = rec
+    return blob + body, dict(nodes=len(nodes), levels=len(lev) - 1, slots=nslots, sets=n_sets,
+                             max_stage=max_stage, nout=nout)
+
+
+def write(path):
+    out = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.",
+           "// Per-class tables of the nuclear-gradient kernel (layout: GradHdr in qk_internal.h).",
+           "#pragma once", "namespace qk {"]
+    names = []
+    for cls in quartet_classes():
+        t = tag(cls)
+        blob, info = class_blob(cls)
+        out.append(f"static const int kGradBlob_{t}[{len(blob)}] = {{")
+        for i in range(0, len(blob), 24):
+            out.append("    " + ", ".join(str(v) for v in blob[i:i + 24]) + ",")
+        out.append("};")
+        names.append((t, len(blob)))
+    out.append("static const int* const kGradBlobs[21] = {" + ", ".join(f"kGradBlob_{t}" for t, _ in names) + "};")
+    out.append("static const int kGradBlobLen[21] = {" + ", ".join(str(n) for _, n in names) + "};")
+    out.append("}  // namespace qk")
+    new = "\n".join(out) + "\n"
+    if not os.path.exists(path) or open(path).read() != new:
+        open(path, "w").write(new)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        write(sys.argv[1])
+    else:
+        for cls in quartet_classes():
+            _, info = class_blob(cls)
+            print(tag(cls), info)

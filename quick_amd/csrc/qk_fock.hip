@@ -9,11 +9,12 @@
 // fixed point (2^-48 units); integer atomics make the result independent of
 // the order quartets finish in, so every build is bitwise reproducible.
 //
-// Work layout: the quartet space of a class is a list of "combos" (bra group x
-// ket group, see qk_internal.h) with per-bra prefix offsets; a flat thread
-// index is decoded with two binary searches.  Consecutive lanes of a wavefront
-// therefore walk consecutive kets of the same bra: the bra's pair and primitive
-// data are broadcast loads and the ket's are coalesced.
+// Work layout: the host cuts each class's quartet space into tiles {bra pair,
+// first ket, length <= 64} (bra and kets from the class- and bound-sorted pair
+// lists, so the kets of a tile are consecutive and all pass the Schwarz test).
+// A wavefront takes one tile at a time: the bra's pair and primitive data are
+// broadcast loads and the kets' primitive-major data are coalesced.  The
+// heavy d classes use the wavefront-cooperative kernel (qk_team.h) instead.
 #include "qk_kernels.h"
 
 namespace qk {
@@ -24,12 +25,15 @@ namespace qk {
 // Dp = diag(cons) D diag(cons): folds QUICK's per-function cons factors
 // (applied inside hrrwhole, subs/hrr.f90:261-295) into the density; the
 // operator is scaled back by the same factors in fock_finalize.
+// Dpf (optional) receives the same matrix rounded to FP32 for the FP32 kernels.
 __global__ void scale_density(const double* __restrict__ D, const double* __restrict__ cons, double* __restrict__ Dp,
-                              int n) {
+                              float* __restrict__ Dpf, int n) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)n * n) return;
     const int i = int(k / n), j = int(k % n);
-    Dp[k] = cons[i] * cons[j] * D[k];
+    const double v = cons[i] * cons[j] * D[k];
+    Dp[k] = v;
+    if (Dpf) Dpf[k] = float(v);
 }
 
 // cutmatrix(II,JJ) = max |D| over the (II,JJ) shell block (schwarz.f90:550-599)
@@ -110,6 +114,28 @@ static const ScratchQuery kFockScratch[kNumQuartetClasses] = {
     fock_scratch_dppp, fock_scratch_dpds, fock_scratch_dpdp, fock_scratch_ddss, fock_scratch_ddps, fock_scratch_ddpp,
     fock_scratch_ddds, fock_scratch_dddp, fock_scratch_dddd};
 
+#define QK_DECL(T) void launch_fock_f32_##T(const FockArgs&, const ClassWork&, int, hipStream_t); size_t fock_scratch_f32_##T();
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
+QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
+QK_DECL(ddss) QK_DECL(ddps)
+#undef QK_DECL
+// FP32 kernels of the mixed-precision mode; the cooperative classes stay FP64
+static const FockLauncher kFockF32Launchers[kNumQuartetClasses] = {
+    launch_fock_f32_ssss, launch_fock_f32_psss, launch_fock_f32_psps, launch_fock_f32_ppss, launch_fock_f32_ppps,
+    launch_fock_f32_pppp, launch_fock_f32_dsss, launch_fock_f32_dsps, launch_fock_f32_dspp, launch_fock_f32_dsds,
+    launch_fock_f32_dpss, launch_fock_f32_dpps, launch_fock_f32_dppp, launch_fock_f32_dpds, nullptr,
+    launch_fock_f32_ddss, launch_fock_f32_ddps, nullptr, nullptr, nullptr, nullptr};
+static const ScratchQuery kFockF32Scratch[kNumQuartetClasses] = {
+    fock_scratch_f32_ssss, fock_scratch_f32_psss, fock_scratch_f32_psps, fock_scratch_f32_ppss, fock_scratch_f32_ppps,
+    fock_scratch_f32_pppp, fock_scratch_f32_dsss, fock_scratch_f32_dsps, fock_scratch_f32_dspp, fock_scratch_f32_dsds,
+    fock_scratch_f32_dpss, fock_scratch_f32_dpps, fock_scratch_f32_dppp, fock_scratch_f32_dpds, nullptr,
+    fock_scratch_f32_ddss, fock_scratch_f32_ddps, nullptr, nullptr, nullptr, nullptr};
+bool fock_class_has_f32(int k) { return kFockF32Launchers[k] != nullptr; }
+void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+    kFockF32Launchers[k](a, w, grid, s);
+}
+size_t fock_class_scratch_f32(int k) { return kFockF32Scratch[k] ? kFockF32Scratch[k]() : 0; }
+
 void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
     kFockLaunchers[k](a, w, grid, s);
 }
@@ -122,8 +148,8 @@ void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, con
 
 static inline int blocks_for(long long count, int bs) { return int((count + bs - 1) / bs); }
 
-void launch_scale_density(const double* D, const double* cons, double* Dp, int n, hipStream_t s) {
-    hipLaunchKernelGGL(scale_density, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, D, cons, Dp, n);
+void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s) {
+    hipLaunchKernelGGL(scale_density, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, D, cons, Dp, Dpf, n);
 }
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s) {

@@ -1,0 +1,296 @@
+// qk_grad.hip -- four-centre part of the nuclear gradient on MI355X (gfx950).
+//
+// dE2/dR at fixed density, E2 = 1/2 sum D G(D), over the same class- and
+// bound-sorted Qshell pair lists as the Fock build.  Per surviving quartet
+// (Y_ij Y_kl > cutoff and Y_ij Y_kl DNmax > cutoff, one-centre quartets
+// skipped; gradient.f90:289-316) one wavefront:
+//   1. runs the primitive loop: QUICK's Boys fit asked for order
+//      sum(Qfinal) + 2 (shell.f90:1906-1912), the VRR raised by one on both
+//      sides as a levelled node program in LDS, and accumulates four
+//      contracted sets: S0 (weight 1), SA/SB/SC (weights 2 alpha, 2 beta,
+//      2 gamma; classopt, shell.f90:2056-2114);
+//   2. forms the density factor of every function quartet,
+//      K = w (4 D_ab D_cd - h (D_ac D_bd + D_ad D_bc)), with the quartet's
+//      symmetry weight w and exchange weight h;
+//   3. for each derivative term (A+, A-, B+, B-, C+, C-) applies the bra and
+//      ket HRR tables to the source set and contracts the result with K:
+//      d/dA_x (ab|cd) = 2 alpha (a+1x b|cd) - a_x (a-1x b|cd), likewise for B
+//      and C (hrrwholeopt, subs/hrr.f90:862-1014);
+//   4. reduces the nine sums over the wavefront in a fixed order and adds
+//      them, and D = -(A + B + C) (classopt, shell.f90:2463-2520), to the
+//      int64 fixed-point gradient with integer atomics (reproducible).
+// The class tables come from gen_grad.py (layout GradHdr in qk_internal.h).
+#include "qk_kernels.h"
+
+namespace qk {
+
+// QUICK's FmT (subs/fmt.f90:12-84) with the number of values a runtime count
+// nf <= kMaxF: the same branches and recursions as boys_ref, F_0..F_nf
+// produced (loops unrolled to kMaxF so that F stays in registers).
+constexpr int kMaxF = 10;
+__device__ __forceinline__ void boys_ref_n(double X, int Mb, int nf, double* F) {
+    const double E = exp(-X);
+    const bool up = (X > 3.0e-5) || (X > 3.0e-7 && Mb < 4);
+    if (up) {
+        double G[1];
+        boys_ref<0>(X, Mb, G);
+        F[0] = G[0];
+        const double XINV = 1.0 / X;
+#pragma unroll
+        for (int m = 1; m <= kMaxF; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
+    } else {
+        double v = (1.0 - X) / double(2 * Mb + 1);
+        const double twot = X * 2.0;
+        for (int m = Mb - 1; m >= nf; --m) v = (twot * v + E) / double(m * 2 + 1);
+        double cur = v;
+#pragma unroll
+        for (int m = kMaxF; m >= 0; --m) {
+            if (m < nf) cur = (twot * cur + E) / double(m * 2 + 1);
+            F[m] = cur;
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, double v) {
+    const long long iv = __double2ll_rn(v * 0x1p44);
+    if (iv != 0) atomicAdd(G + idx, (unsigned long long)iv);
+}
+
+__global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int n = a.n;
+    const int nSlots = tab[kGhSlots], nLevels = tab[kGhLevels], LT2 = tab[kGhLT] + 2;
+    const int n0 = tab[kGhN0], nA = tab[kGhNA], nB = tab[kGhNB], nC = tab[kGhNC];
+    const int nS = n0 + nA + nB + nC;
+    const int NOUT = tab[kGhNOut], CA = tab[kGhCA], CB = tab[kGhCB], CC = tab[kGhCC], CD = tab[kGhCD];
+    const int nStages = tab[kGhStages];
+    const int RV = nSlots > tab[kGhMaxStage] ? nSlots : tab[kGhMaxStage];
+    const uint4* nodes = reinterpret_cast<const uint4*>(tab + tab[kGhOffNodes]);
+    const int* lev = tab + tab[kGhOffLev];
+    const int* tgt = tab + tab[kGhOffTgt];
+    double* V = lds;              // VRR slots; after the primitive loop the stage buffers
+    double* S = lds + RV;         // S0 | SA | SB | SC
+    double* Kv = S + nS;          // density factor per function quartet
+    double* pw = Kv + NOUT;       // pw[axis*4+k] = AB_axis^k, pw[12+axis*4+k] = CD_axis^k
+    double* Fl = pw + 24;
+    const double X0 = 34.98683665524972;
+    unsigned long long nsurv = 0, nprimq = 0;
+
+    for (long long t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
+        const Tile tl = w.tiles[t];
+        const int p = tl.p;
+        const int qa = a.pr.qa[p], qb = a.pr.qb[p];
+        const int sA = a.q.shell[qa], sB = a.q.shell[qb];
+        for (int qq = 0; qq < tl.len; ++qq) {
+            const int q = tl.q0 + qq;
+            const int qc = a.pr.qa[q], qd = a.pr.qb[q];
+            const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
+            if (atA == atB && atA == atC && atA == atD) continue;   // translational invariance
+            const int sC = a.q.shell[qc], sD = a.q.shell[qd];
+            const double ct = a.pr.Y[q] * a.pr.Y[p];
+            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
+            if (!(ct > a.cutoff && ct * dn > a.cutoff)) continue;   // uniform across the wavefront
+            ++nsurv;
+            const int Mb = a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd] + 2;
+            const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
+            const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+            __syncthreads();                       // previous quartet's readers of S/V are done
+            for (int k = lane; k < nS; k += 64) S[k] = 0.0;
+            if (lane == 0) V[0] = 0.0;
+
+            const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+            const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
+            for (int i = 0; i < nbp; ++i) {
+                const int bi = pb0 + i * pbs;
+                const double zab = a.pp.zeta[bi], oo2z = a.pp.oo2z[bi];
+                const double Px = a.pp.Px[bi], Py = a.pp.Py[bi], Pz = a.pp.Pz[bi];
+                const double X2 = X0 * a.pp.K[bi];
+                const double cp1 = dn * a.pp.cut[bi];
+                const double wA = 2.0 * a.ea[bi], wB = 2.0 * a.eb[bi];
+                for (int j = 0; j < nkp; ++j) {
+                    const int kj = pk0 + j * pks;
+                    if (!(cp1 * a.pp.cut[kj] > a.cutoff)) continue;
+                    ++nprimq;
+                    const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
+                    const double wC = 2.0 * a.ea[kj];
+                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
+                    const double inv = 1.0 / (zab + zcd);
+                    const double roz = zcd * inv, roe = zab * inv;
+                    const double ROU = zab * roz;
+                    const double Wx = (Px * zab + Qx * zcd) * inv;
+                    const double Wy = (Py * zab + Qy * zcd) * inv;
+                    const double Wz = (Pz * zab + Qz * zcd) * inv;
+                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    double F[kMaxF + 1];
+                    boys_ref_n(T, Mb, LT2, F);
+                    const double s = X2 * a.pp.K[kj] * sqrt(inv);
+                    double fsel = F[0] * s;
+#pragma unroll
+                    for (int m = 1; m <= kMaxF; ++m) fsel = (lane == m) ? F[m] * s : fsel;
+                    __syncthreads();
+                    if (lane <= LT2) Fl[lane] = fsel;
+                    const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
+                    const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
+                    const double oo2ze = 0.5 * inv;
+                    __syncthreads();
+                    for (int k = lev[0] + lane; k < lev[1]; k += 64) {
+                        const uint4 nd = nodes[k];
+                        V[nd.x & 0xffffu] = Fl[(nd.w >> 8) & 0xffu];
+                    }
+                    for (int L = 1; L < nLevels; ++L) {
+                        __syncthreads();
+                        for (int k = lev[L] + lane; k < lev[L + 1]; k += 64) {
+                            const uint4 nd = nodes[k];
+                            const unsigned fl = nd.w;
+                            const bool bra = (fl & 3u) == 1u;
+                            const int dir = (fl >> 2) & 3;
+                            const double n1 = double((fl >> 4) & 15u), ne = double((fl >> 8) & 15u);
+                            const double pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
+                            const double wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
+                            const double c3 = n1 * (bra ? oo2z : oo2e);
+                            const double c4 = bra ? roz : roe;
+                            const double v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
+                                             c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
+                            V[nd.x & 0xffffu] = v;
+                        }
+                    }
+                    __syncthreads();
+                    for (int k = lane; k < nS; k += 64) {
+                        const double wk = k < n0 ? 1.0 : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
+                        S[k] += wk * V[tgt[k]];
+                    }
+                }
+            }
+            __syncthreads();
+
+            // powers of A-B and C-D; density factors
+            if (lane < 24) {
+                const int ax = (lane % 12) >> 2, k = lane & 3;
+                const int u = lane < 12 ? atA : atC, v = lane < 12 ? atB : atD;
+                const double d = a.xyz[3 * u + ax] - a.xyz[3 * v + ax];
+                double r = 1.0;
+                for (int e = 0; e < k; ++e) r *= d;
+                pw[lane] = r;
+            }
+            {
+                const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
+                double wq = 1.0;
+                if (qa == qb) wq *= 0.5;
+                if (qc == qd) wq *= 0.5;
+                if (p == q) wq *= 0.5;
+                for (int o = lane; o < NOUT; o += 64) {
+                    const int id = o % CD, ic = (o / CD) % CC, ib = (o / (CD * CC)) % CB, ia = o / (CD * CC * CB);
+                    const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
+                    const double Dab = a.Dp[ra + fb + ib], Dcd = a.Dp[(long long)(fc + ic) * n + fd + id];
+                    const double Dac = a.Dp[ra + fc + ic], Dbd = a.Dp[rb + fd + id];
+                    const double Dad = a.Dp[ra + fd + id], Dbc = a.Dp[rb + fc + ic];
+                    Kv[o] = wq * (4.0 * Dab * Dcd - a.hybrid * (Dac * Dbd + Dad * Dbc));
+                }
+            }
+            double g[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g[k] = 0.0;
+            for (int st = 0; st < nStages; ++st) {
+                const int* rec = tab + kGhSize + st * kGradStageFields;
+                const int dir = rec[0], src = rec[1], nR1 = rec[2], nCol = rec[3], nR2 = rec[4];
+                const int* bt = tab + rec[5];
+                const int* kt = tab + rec[6];
+                const int* con = tab + rec[7];
+                const double* Ss = S + (src == 0 ? 0 : (src == 1 ? n0 : (src == 2 ? n0 + nA : n0 + nA + nB)));
+                double* o1 = V;
+                double* o2 = V + nR1 * nCol;
+                __syncthreads();                   // pw, Kv, previous stage's readers
+                for (int idx = lane; idx < nR1 * nCol; idx += 64) {
+                    const int r1 = idx / nCol, col = idx - r1 * nCol;
+                    const int* tr = bt + r1 * (3 * kGradHrrTerms);
+                    double v = 0.0;
+                    for (int u = 0; u < kGradHrrTerms; ++u) {
+                        const int cf = tr[3 * u + 2];
+                        if (cf == 0) break;
+                        const int pc = tr[3 * u + 1];
+                        v += double(cf) * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] *
+                             Ss[tr[3 * u] * nCol + col];
+                    }
+                    o1[idx] = v;
+                }
+                __syncthreads();
+                for (int idx = lane; idx < nR1 * nR2; idx += 64) {
+                    const int r1 = idx / nR2, r2 = idx - r1 * nR2;
+                    const int* tr = kt + r2 * (3 * kGradHrrTerms);
+                    double v = 0.0;
+                    for (int u = 0; u < kGradHrrTerms; ++u) {
+                        const int cf = tr[3 * u + 2];
+                        if (cf == 0) break;
+                        const int pc = tr[3 * u + 1];
+                        v += double(cf) * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] *
+                             o1[r1 * nCol + tr[3 * u]];
+                    }
+                    o2[idx] = v;
+                }
+                __syncthreads();
+                for (int o = lane; o < NOUT; o += 64) {
+                    const double K = Kv[o];
+                    const int* cr = con + 6 * o;
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) {
+                        const int ix = cr[2 * x];
+                        if (ix >= 0) {
+                            const double v = double(cr[2 * x + 1]) * K * o2[ix];
+                            if (dir == 0) g[x] += v; else if (dir == 1) g[3 + x] += v; else g[6 + x] += v;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) g[k] = wave_sum_f64(g[k]);
+            if (lane < 3) {
+                const double gA = lane == 0 ? g[0] : (lane == 1 ? g[1] : g[2]);
+                const double gB = lane == 0 ? g[3] : (lane == 1 ? g[4] : g[5]);
+                const double gC = lane == 0 ? g[6] : (lane == 1 ? g[7] : g[8]);
+                gx_add(a.G, 3LL * atA + lane, gA);
+                gx_add(a.G, 3LL * atB + lane, gB);
+                gx_add(a.G, 3LL * atC + lane, gC);
+                gx_add(a.G, 3LL * atD + lane, -(gA + gB + gC));
+            }
+        }
+    }
+    if (lane == 0) {
+        if (nsurv) atomicAdd(a.counters + 0, nsurv);
+        if (nprimq) atomicAdd(a.counters + 1, nprimq);
+    }
+}
+
+// grad <- grad + G * 2^-44
+__global__ void grad_finalize(const unsigned long long* __restrict__ G, double* __restrict__ grad, int m) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) grad[k] += (double)(long long)G[k] * 0x1p-44;
+}
+
+size_t grad_lds_bytes(const int* hdr) {
+    const int RV = hdr[kGhSlots] > hdr[kGhMaxStage] ? hdr[kGhSlots] : hdr[kGhMaxStage];
+    const int nS = hdr[kGhN0] + hdr[kGhNA] + hdr[kGhNB] + hdr[kGhNC];
+    return sizeof(double) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16);
+}
+
+int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s) {
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(grad_kernel, dim3(grid), dim3(64), lds, s, a, w, dtab);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s) {
+    hipLaunchKernelGGL(grad_finalize, dim3((m + 255) / 256), dim3(256), 0, s, G, grad, m);
+}
+
+}  // namespace qk

@@ -17,13 +17,20 @@
 #include "../../include/quick_mi355x.h"
 #include "qk_internal.h"
 #include "eri_flops.h"
+#include "grad_tables.inc"
 
 namespace qk {
 void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 size_t fock_class_scratch(int k);
 bool fock_class_is_team(int k);
+bool fock_class_has_f32(int k);
+void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
+size_t fock_class_scratch_f32(int k);
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
-void launch_scale_density(const double* D, const double* cons, double* Dp, int n, hipStream_t s);
+void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
+size_t grad_lds_bytes(const int* hdr);
+int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s);
+void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
@@ -109,10 +116,31 @@ static int pair_class_of(int la, int lb) {
 }
 
 struct ClassHost {
-    std::vector<Tile> tiles;
-    long long total = 0;   // quartets passing the Schwarz test
-    DBuf<Tile> d_tiles;
+    std::vector<Tile> tiles;     // FP64 work
+    std::vector<Tile> tiles32;   // FP32 work (mixed precision only)
+    long long total = 0;         // quartets passing the Schwarz test
+    long long total32 = 0;       // of which in the FP32 lists
+    DBuf<Tile> d_tiles, d_tiles32;
 };
+
+// Boys values F_m(k / kBoysTabPerUnit) for the FP32 kernels, from the series
+// F_m(T) = exp(-T) sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)) in FP64 (all terms
+// positive, so it is accurate to a few ulp over the whole table).
+static void boys_table(std::vector<float>& tab) {
+    tab.assign((size_t)kBoysTabRows * kBoysTabM, 0.0f);
+    for (int k = 0; k < kBoysTabRows; ++k) {
+        const double T = double(k) / kBoysTabPerUnit;
+        for (int m = 0; m < kBoysTabM; ++m) {
+            double term = 1.0 / (2 * m + 1), sum = term;
+            for (int i = 1; i < 1000; ++i) {
+                term *= 2.0 * T / (2 * m + 2 * i + 1);
+                sum += term;
+                if (term < 1e-17 * sum) break;
+            }
+            tab[(size_t)k * kBoysTabM + m] = float(std::exp(-T) * sum);
+        }
+    }
+}
 
 struct qk_ctx {
     int device = 0;
@@ -134,16 +162,36 @@ struct qk_ctx {
     std::vector<int> ap_Pi, ap_Pj;   // global primitive indices, lower shell first (cutprim orientation)
     std::vector<double> ap_Y;        // Schwarz bound of the parent shell pair
     std::vector<double> ap_zeta, ap_oo2z, ap_Px, ap_Py, ap_Pz, ap_K, ap_cut;  // per primitive pair
+    std::vector<double> ap_ea, ap_eb;   // exponents of the qa and qb primitives
     DBuf<int> dap_qa, dap_qb, dap_prim0, dap_nprim;
     // the kernels' primitive-major copy of the significant pairs' primitive pairs
     DBuf<int> dp_pstride;
     DBuf<double> dk_zeta, dk_oo2z, dk_Px, dk_Py, dk_Pz, dk_K, dk_cut;
+    DBuf<float> dkf_zeta, dkf_oo2z, dkf_Px, dkf_Py, dkf_Pz, dkf_K, dkf_cut;   // FP32 copy
+    DBuf<float> d_boys_tab, d_Dpf;
+    DBuf<double> dk_ea, dk_eb;   // exponents, primitive-major like dk_*
 
     // prepared work lists and the options they were built for
     bool prepared = false;
-    double prep_tau = -1, prep_pair_cut = -1;
+    double prep_tau = -1, prep_pair_cut = -1, prep_fp32_thr = -1;
+    int prep_precision = -1;
     int nclass_pairs = 0;
     std::vector<ClassHost> classes;
+    // the pair groups of the prepared lists (class, first pair, count) and the
+    // pairs' Schwarz bounds, kept for the gradient's work lists
+    std::vector<int> grp_cls, grp_start, grp_count;
+    std::vector<double> pair_Y;
+
+    // gradient: per-class tables on the device, work lists, accumulators
+    DBuf<int> d_gtab[kNumQuartetClasses];
+    std::vector<Tile> gtiles[kNumQuartetClasses];
+    DBuf<Tile> d_gtiles[kNumQuartetClasses];
+    double grad_prep_cutoff = -1;
+    bool grad_prepared = false;
+    DBuf<unsigned long long> d_gacc, d_gcounters;
+    long long gcls_quartets[kNumQuartetClasses] = {};
+    long long gcls_prims[kNumQuartetClasses] = {};
+    DBuf<double> d_grad;
 
     // device data
     DBuf<double> d_xyz, d_cons;
@@ -161,12 +209,17 @@ struct qk_ctx {
     long long cls_quartets[kNumQuartetClasses] = {};
     long long cls_prims[kNumQuartetClasses] = {};
     double cls_flops[kNumQuartetClasses] = {};
+    long long cls_quartets32[kNumQuartetClasses] = {};
+    long long cls_prims32[kNumQuartetClasses] = {};
+    double cls_flops32[kNumQuartetClasses] = {};
     // optional per-class kernel timing (HIP events on the launch stream)
     bool timing = false;
-    hipEvent_t ev_t0[kNumQuartetClasses] = {}, ev_t1[kNumQuartetClasses] = {};
-    bool cls_timed[kNumQuartetClasses] = {};
-    double cls_ms_sum[kNumQuartetClasses] = {};
-    int cls_ms_n[kNumQuartetClasses] = {};
+    // (slot k: the FP64 launch of class k; slot kNumQuartetClasses + k: its FP32 launch)
+    static constexpr int kSlots = 2 * kNumQuartetClasses;
+    hipEvent_t ev_t0[kSlots] = {}, ev_t1[kSlots] = {};
+    bool cls_timed[kSlots] = {};
+    double cls_ms_sum[kSlots] = {};
+    int cls_ms_n[kSlots] = {};
 
     // reference-ABI state
     int method = 0;
@@ -174,6 +227,7 @@ struct qk_ctx {
     std::vector<double> abi_cutmatrix;
     bool abi_have_cutmatrix = false;
     bool abi_have_dense = false;
+    double abi_grad_cutoff = 1e-11;
 };
 
 extern "C" void qk_default_opts(qk_opts* o) {
@@ -211,7 +265,7 @@ extern "C" qk_ctx* qk_create(int device) {
         }
     (void)hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming);
     for (auto& e : c->ev_cls) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    if (c->d_counters.alloc(2 * kNumQuartetClasses + 2)) {
+    if (c->d_counters.alloc(kNumCounters)) {
         delete c;
         return nullptr;
     }
@@ -225,7 +279,7 @@ extern "C" void qk_destroy(qk_ctx* c) {
     for (auto& s : c->streams)
         if (s) (void)hipStreamDestroy(s);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
+    for (int k = 0; k < qk_ctx::kSlots; ++k) {
         if (c->ev_t0[k]) (void)hipEventDestroy(c->ev_t0[k]);
         if (c->ev_t1[k]) (void)hipEventDestroy(c->ev_t1[k]);
     }
@@ -243,7 +297,7 @@ static int build_all_pairs(qk_ctx* c) {
     const int nq = (int)c->q_shell.size();
     c->ap_qa.clear(); c->ap_qb.clear(); c->ap_cls.clear(); c->ap_prim0.clear(); c->ap_nprim.clear();
     c->ap_Pi.clear(); c->ap_Pj.clear();
-    std::vector<double> zeta, Px, Py, Pz, K;
+    std::vector<double> zeta, Px, Py, Pz, K, ea, eb;
     for (int i = 0; i < nq; ++i)
         for (int j = i; j < nq; ++j) {
             int qa = i, qb = j;
@@ -264,6 +318,8 @@ static int build_all_pairs(qk_ctx* c) {
                     const double a = c->gcexpo[u + 10 * (size_t)ca], b = c->gcexpo[v + 10 * (size_t)cb];
                     const double kp = std::exp(-a * b / (a + b) * (DAB * DAB)) / (a + b);
                     zeta.push_back(a + b);
+                    ea.push_back(a);
+                    eb.push_back(b);
                     Px.push_back((A[0] * a + B[0] * b) / (a + b));
                     Py.push_back((A[1] * a + B[1] * b) / (a + b));
                     Pz.push_back((A[2] * a + B[2] * b) / (a + b));
@@ -277,7 +333,7 @@ static int build_all_pairs(qk_ctx* c) {
     c->ap_Y.assign(c->ap_qa.size(), 0.0);
     std::vector<double> oo2z(zeta.size());
     for (size_t k = 0; k < zeta.size(); ++k) oo2z[k] = 0.5 / zeta[k];
-    c->ap_zeta = zeta; c->ap_oo2z = oo2z; c->ap_Px = Px; c->ap_Py = Py; c->ap_Pz = Pz; c->ap_K = K;
+    c->ap_zeta = zeta; c->ap_oo2z = oo2z; c->ap_ea = ea; c->ap_eb = eb; c->ap_Px = Px; c->ap_Py = Py; c->ap_Pz = Pz; c->ap_K = K;
     c->ap_cut.assign(zeta.size(), 0.0);
     (void)hipSetDevice(c->device);
     if (c->dpp_zeta.upload(zeta) || c->dpp_oo2z.upload(oo2z) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
@@ -393,7 +449,18 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         set_err("prepare: system and Schwarz bounds must be uploaded first");
         return -1;
     }
-    if (c->prepared && c->prep_tau == o->tau && c->prep_pair_cut == o->pair_cut) return 0;
+    if (o->precision != QK_FP64 && o->precision != QK_MIXED) {
+        set_err("precision must be QK_FP64 or QK_MIXED (got %d)", o->precision);
+        return -4;
+    }
+    const bool mixed = o->precision == QK_MIXED;
+    if (mixed && !(o->fp32_threshold >= 0.0)) {
+        set_err("fp32_threshold must be >= 0 (got %g)", o->fp32_threshold);
+        return -4;
+    }
+    if (c->prepared && c->prep_tau == o->tau && c->prep_pair_cut == o->pair_cut &&
+        c->prep_precision == o->precision && (!mixed || c->prep_fp32_thr == o->fp32_threshold))
+        return 0;
     double Ymax = 0.0;
     for (double y : c->Ycut) Ymax = std::max(Ymax, y);
 
@@ -429,7 +496,7 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     }
     // primitive-major copy per group: entry (pair k, primitive j) at
     // base(group) + j * count(group) + (k - start(group))
-    std::vector<double> kz, ko, kx, ky, kzz, kK, kc;
+    std::vector<double> kz, ko, kx, ky, kzz, kK, kc, kea, keb;
     for (const Group& g : groups) {
         const int base = (int)kz.size();
         const int np = c->ap_nprim[sel[g.start]];
@@ -439,6 +506,7 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                 kz.push_back(c->ap_zeta[src]); ko.push_back(c->ap_oo2z[src]);
                 kx.push_back(c->ap_Px[src]); ky.push_back(c->ap_Py[src]); kzz.push_back(c->ap_Pz[src]);
                 kK.push_back(c->ap_K[src]); kc.push_back(c->ap_cut[src]);
+                kea.push_back(c->ap_ea[src]); keb.push_back(c->ap_eb[src]);
             }
         for (int k = g.start; k < g.start + g.count; ++k) {
             prim0[k] = base + (k - g.start);
@@ -454,7 +522,7 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         ClassHost& CH = c->classes[k];
-        long long total = 0;
+        long long total = 0, total32 = 0;
         for (size_t gb = 0; gb < groups.size(); ++gb) {
             if (groups[gb].cls != cb) continue;
             for (size_t gk = 0; gk < groups.size(); ++gk) {
@@ -472,30 +540,67 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                         if (yp * Yp[G2.start + mid] > o->tau) lo = mid + 1; else hi = mid;
                     }
                     const int q_begin = same ? i : 0;
+                    // mixed precision: kets with Y_ij*Y_kl >= threshold stay FP64,
+                    // the rest of the prefix goes to the FP32 list
+                    int lo64 = lo;
+                    if (mixed && fock_class_has_f32(k)) {
+                        int a0 = 0, a1 = lo;
+                        while (a0 < a1) {
+                            const int mid = (a0 + a1) >> 1;
+                            if (yp * Yp[G2.start + mid] >= o->fp32_threshold) a0 = mid + 1; else a1 = mid;
+                        }
+                        lo64 = std::max(a0, q_begin);
+                    }
                     // a wavefront of the cooperative kernel works through its tile's
                     // quartets one after another: short tiles spread them over the GPU
                     const int chunk = fock_class_is_team(k) ? 2 : 64;
-                    for (int q = q_begin; q < lo; q += chunk)
-                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
+                    for (int q = q_begin; q < lo64; q += chunk)
+                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo64 - q), 0});
+                    for (int q = lo64; q < lo; q += 64)
+                        CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(64, lo - q), 0});
                     if (lo > q_begin) total += lo - q_begin;
+                    if (lo > lo64) total32 += lo - lo64;
                 }
             }
         }
         CH.total = total;
+        CH.total32 = total32;
     }
 
     (void)hipSetDevice(c->device);
     if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_pstride.upload(pstride) ||
         c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dk_zeta.upload(kz) || c->dk_oo2z.upload(ko) ||
         c->dk_Px.upload(kx) || c->dk_Py.upload(ky) || c->dk_Pz.upload(kzz) || c->dk_K.upload(kK) ||
-        c->dk_cut.upload(kc))
+        c->dk_cut.upload(kc) || c->dk_ea.upload(kea) || c->dk_eb.upload(keb))
         return -1;
+    c->grp_cls.clear(); c->grp_start.clear(); c->grp_count.clear();
+    for (const Group& g : groups) {
+        c->grp_cls.push_back(g.cls);
+        c->grp_start.push_back(g.start);
+        c->grp_count.push_back(g.count);
+    }
+    c->pair_Y = Yp;
+    c->grad_prepared = false;
     for (auto& CH : c->classes)
-        if (CH.d_tiles.upload(CH.tiles)) return -1;
+        if (CH.d_tiles.upload(CH.tiles) || CH.d_tiles32.upload(CH.tiles32)) return -1;
+    if (mixed) {
+        auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+        if (c->dkf_zeta.upload(f32(kz)) || c->dkf_oo2z.upload(f32(ko)) || c->dkf_Px.upload(f32(kx)) ||
+            c->dkf_Py.upload(f32(ky)) || c->dkf_Pz.upload(f32(kzz)) || c->dkf_K.upload(f32(kK)) ||
+            c->dkf_cut.upload(f32(kc)) || c->d_Dpf.alloc((size_t)c->nbasis * c->nbasis))
+            return -1;
+        if (!c->d_boys_tab.p) {
+            std::vector<float> tab;
+            boys_table(tab);
+            if (c->d_boys_tab.upload(tab)) return -1;
+        }
+    }
     c->nclass_pairs = npair;
     c->prepared = true;
     c->prep_tau = o->tau;
     c->prep_pair_cut = o->pair_cut;
+    c->prep_precision = o->precision;
+    c->prep_fp32_thr = o->fp32_threshold;
     return 0;
 }
 
@@ -515,6 +620,9 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
     a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
     a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
+    a.ppf = {c->dkf_zeta.p, c->dkf_oo2z.p, c->dkf_Px.p, c->dkf_Py.p, c->dkf_Pz.p, c->dkf_K.p, c->dkf_cut.p};
+    a.Dpf = c->d_Dpf.p;
+    a.boys_tab = c->d_boys_tab.p;
     a.xyz = c->d_xyz.p;
     a.Dp = c->d_Dp.p;
     a.cutmat = c->d_cm.p;
@@ -534,17 +642,18 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                         hipStream_t s) {
     if (prepare(c, o)) return -1;
     const int n = c->nbasis, ns = c->nshell;
-    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, n, s);
+    const bool mixed = o->precision == QK_MIXED;
+    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, mixed ? c->d_Dpf.p : nullptr, n, s);
     if (dcutmat_or_null)
         HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
     launch_fill_u64(c->d_X.p, (long long)n * n, s);
-    launch_fill_u64(c->d_counters.p, 2 * kNumQuartetClasses + 2, s);
+    launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);
     // fold the previous build's kernel times into the running sums (waits for it)
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
+    for (int k = 0; k < qk_ctx::kSlots; ++k) {
         if (c->cls_timed[k]) {
             float ms = 0.0f;
             HIPCHK(hipEventSynchronize(c->ev_t1[k]));
@@ -558,6 +667,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     std::vector<int> order(kNumQuartetClasses);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int x, int y) { return c->classes[x].total > c->classes[y].total; });
+    if (mixed && !c->d_boys_tab.p) { set_err("mixed precision: Boys table missing"); return -1; }
     // All classes run one after another on the caller's stream (one hardware
     // queue: kernels that spill to scratch never run concurrently with others).
     // Kernels that spill get a grid small enough that their private-segment
@@ -566,30 +676,40 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     constexpr double kScratchBudget = 256.0 * 1024 * 1024;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
-        if (CH.tiles.empty()) continue;
-        ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size()};
-        a.counters = c->d_counters.p + 2 * k;
-        if (fock_class_is_team(k)) {
-            // one 64-lane block per tile; up to 16 resident per CU
-            const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
+        if (CH.tiles.empty() && CH.tiles32.empty()) continue;
+        if (!CH.tiles.empty()) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
-            launch_team_class(k, a, w, grid, s);
+            ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size()};
+            a.counters = c->d_counters.p + 2 * k;
+            if (fock_class_is_team(k)) {
+                // one 64-lane block per tile; up to 16 resident per CU
+                const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
+                launch_team_class(k, a, w, grid, s);
+            } else {
+                const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
+                long long cap = 256LL * 16;
+                const size_t scratch = fock_class_scratch(k);
+                if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+                launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), s);
+            }
             HIPCHK(hipGetLastError());
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
             c->cls_timed[k] = c->timing;
-            continue;
         }
-        const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
-        long long cap = 256LL * 16;
-        const size_t scratch = fock_class_scratch(k);
-        if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
-        const int grid = (int)std::min<long long>(want, cap);
-        a.counters = c->d_counters.p + 2 * k;
-        if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
-        launch_fock_class(k, a, w, grid, s);
-        HIPCHK(hipGetLastError());
-        if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
-        c->cls_timed[k] = c->timing;
+        if (mixed && !CH.tiles32.empty()) {
+            ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size()};
+            a.counters = c->d_counters.p + 2 * kNumQuartetClasses + 2 * k;
+            const long long want = ((long long)CH.tiles32.size() + 3) / 4;
+            long long cap = 256LL * 16;
+            const size_t scratch = fock_class_scratch_f32(k);
+            if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+            const int slot = kNumQuartetClasses + k;
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
+            launch_fock_f32_class(k, a, w, (int)std::min<long long>(want, cap), s);
+            HIPCHK(hipGetLastError());
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
+            c->cls_timed[slot] = c->timing;
+        }
     }
     launch_fock_finalize(c->d_X.p, c->d_cons.p, dG, n, s);
     HIPCHK(hipGetLastError());
@@ -600,7 +720,6 @@ extern "C" int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_
     if (!c) { set_err("null context"); return -1; }
     qk_opts d;
     if (!o) { qk_default_opts(&d); o = &d; }
-    if (o->precision != QK_FP64) { set_err("mixed precision is not enabled in this build"); return -4; }
     (void)hipSetDevice(c->device);
     return fock_enqueue(c, dD, nullptr, dG, o, (hipStream_t)stream);
 }
@@ -610,25 +729,36 @@ extern "C" int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_
 // sqrt and divisions counted as one each), the prefactor scaling (LT + 1) and the
 // generated VRR; per contracted quartet the screening (12), the generated HRR and
 // the six J/K digestion contractions (12 per integral).
-static double class_flops(int k, long long nq, long long np) {
+static double class_flops(int k, long long nq, long long np, bool fp32) {
     const int* t = kEriFlopTable[k];
     const int LT = t[0] + t[1] + t[2] + t[3];
-    const double per_prim = 35.0 + (24.0 + 4.0 * LT) + (LT + 1) + t[5];
+    // FP32 Boys: table Taylor (5 terms), exp, and the downward recursion
+    const double boys = fp32 ? 12.0 + 3.0 * LT : 24.0 + 4.0 * LT;
+    const double per_prim = 35.0 + boys + (LT + 1) + t[5];
     const double per_quartet = 12.0 + t[6] + 12.0 * t[7];
     return per_prim * double(np) + per_quartet * double(nq);
 }
 
 static int read_stats(qk_ctx* c) {
-    unsigned long long h[2 * kNumQuartetClasses + 2];
+    unsigned long long h[kNumCounters];
     HIPCHK(hipMemcpy(h, c->d_counters.p, sizeof(h), hipMemcpyDeviceToHost));
     c->last = qk_stats{};
+    const int o32 = 2 * kNumQuartetClasses;
     for (int k = 0; k < kNumQuartetClasses; ++k) {
-        c->cls_quartets[k] = (long long)h[2 * k];
-        c->cls_prims[k] = (long long)h[2 * k + 1];
-        c->cls_flops[k] = class_flops(k, c->cls_quartets[k], c->cls_prims[k]);
-        c->last.quartets += c->cls_quartets[k];
-        c->last.prim_quartets += c->cls_prims[k];
-        c->last.flops_fp64 += c->cls_flops[k];
+        const long long q64 = (long long)h[2 * k], p64 = (long long)h[2 * k + 1];
+        const long long q32 = (long long)h[o32 + 2 * k], p32 = (long long)h[o32 + 2 * k + 1];
+        const double f64 = class_flops(k, q64, p64, false), f32 = class_flops(k, q32, p32, true);
+        c->cls_quartets[k] = q64;
+        c->cls_prims[k] = p64;
+        c->cls_flops[k] = f64;
+        c->cls_quartets32[k] = q32;
+        c->cls_prims32[k] = p32;
+        c->cls_flops32[k] = f32;
+        c->last.quartets += q64 + q32;
+        c->last.quartets_fp32 += q32;
+        c->last.prim_quartets += p64 + p32;
+        c->last.flops_fp64 += f64;
+        c->last.flops_fp32 += f32;
     }
     return 0;
 }
@@ -637,12 +767,12 @@ extern "C" int qk_set_timing(qk_ctx* c, int on) {
     if (!c) { set_err("null context"); return -1; }
     (void)hipSetDevice(c->device);
     if (on && !c->ev_t0[0])
-        for (int k = 0; k < kNumQuartetClasses; ++k) {
+        for (int k = 0; k < qk_ctx::kSlots; ++k) {
             HIPCHK(hipEventCreate(&c->ev_t0[k]));
             HIPCHK(hipEventCreate(&c->ev_t1[k]));
         }
     c->timing = on != 0;
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
+    for (int k = 0; k < qk_ctx::kSlots; ++k) {
         c->cls_timed[k] = false;
         c->cls_ms_sum[k] = 0.0;
         c->cls_ms_n[k] = 0;
@@ -650,8 +780,23 @@ extern "C" int qk_set_timing(qk_ctx* c, int on) {
     return 0;
 }
 
+// average kernel time of a timing slot over every timed build since qk_set_timing(c, 1)
+static int slot_ms(qk_ctx* c, int slot, float* ms) {
+    double sum = c->cls_ms_sum[slot];
+    int cnt = c->cls_ms_n[slot];
+    if (c->cls_timed[slot]) {
+        float ms1 = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms1, c->ev_t0[slot], c->ev_t1[slot]));
+        sum += ms1;
+        cnt += 1;
+    }
+    *ms = cnt ? float(sum / cnt) : 0.0f;
+    return 0;
+}
+
 // per-class figures of the last build: quartets, primitive quartets,
-// algorithmic flops, and (when timing was on) the kernel's elapsed ms
+// algorithmic flops (FP64 and FP32 kernels together), and (when timing was on)
+// the FP64 kernel's elapsed ms
 extern "C" int qk_class_stats(qk_ctx* c, int k, long long* nq, long long* np, double* flops, float* ms) {
     if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
     (void)hipSetDevice(c->device);
@@ -660,17 +805,36 @@ extern "C" int qk_class_stats(qk_ctx* c, int k, long long* nq, long long* np, do
     *nq = c->cls_quartets[k];
     *np = c->cls_prims[k];
     *flops = c->cls_flops[k];
-    // average kernel time over every timed build since qk_set_timing(c, 1)
-    double sum = c->cls_ms_sum[k];
-    int cnt = c->cls_ms_n[k];
-    if (c->cls_timed[k]) {
-        float ms1 = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms1, c->ev_t0[k], c->ev_t1[k]));
-        sum += ms1;
-        cnt += 1;
+    return slot_ms(c, k, ms);
+}
+
+// the FP32 share of qk_class_stats' figures (mixed precision) and the FP32 kernel's ms
+extern "C" int qk_class_stats_fp32(qk_ctx* c, int k, long long* nq, long long* np, double* flops, float* ms) {
+    if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipDeviceSynchronize());
+    if (read_stats(c)) return -1;
+    *nq = c->cls_quartets32[k];
+    *np = c->cls_prims32[k];
+    *flops = c->cls_flops32[k];
+    return slot_ms(c, kNumQuartetClasses + k, ms);
+}
+
+extern "C" int qk_boys_table_f32(float* out, int rows, int cols) {
+    if (rows != kBoysTabRows || cols != kBoysTabM) {
+        set_err("Boys table is %d x %d", kBoysTabRows, kBoysTabM);
+        return -1;
     }
-    *ms = cnt ? float(sum / cnt) : 0.0f;
+    std::vector<float> tab;
+    boys_table(tab);
+    std::copy(tab.begin(), tab.end(), out);
     return 0;
+}
+
+extern "C" void qk_boys_table_shape(int* rows, int* cols, int* per_unit) {
+    *rows = kBoysTabRows;
+    *cols = kBoysTabM;
+    *per_unit = kBoysTabPerUnit;
 }
 
 extern "C" int qk_last_stats(qk_ctx* c, qk_stats* s) {
@@ -768,15 +932,133 @@ extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_o
     return qk_set_schwarz(c, Ymx.data(), cp.data());
 }
 
-extern "C" int qk_grad_device(qk_ctx* c, const double*, double*, const qk_opts*, void*) {
-    set_err("qk_grad_device: not available yet");
-    (void)c;
-    return -5;
+// ---------------------------------------------------------------------------
+// Nuclear gradient (four-centre part)
+
+// Work lists of the gradient: the prepared pair groups, every bra with the
+// prefix of kets passing Y_ij*Y_kl > cutoff, in runs of <= chunk quartets (a
+// wavefront works through a run one quartet after another).
+static int grad_prepare(qk_ctx* c, double cutoff) {
+    if (c->grad_prepared && c->grad_prep_cutoff == cutoff) return 0;
+    (void)hipSetDevice(c->device);
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        if (!c->d_gtab[k].p) {
+            std::vector<int> blob(kGradBlobs[k], kGradBlobs[k] + kGradBlobLen[k]);
+            if (c->d_gtab[k].upload(blob)) return -1;
+        }
+        const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
+        const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
+        const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
+        const int chunk = LT <= 3 ? 16 : (LT <= 5 ? 4 : 1);
+        std::vector<Tile>& T = c->gtiles[k];
+        T.clear();
+        const size_t ng = c->grp_cls.size();
+        for (size_t gb = 0; gb < ng; ++gb) {
+            if (c->grp_cls[gb] != cb) continue;
+            for (size_t gk = 0; gk < ng; ++gk) {
+                if (c->grp_cls[gk] != ck) continue;
+                if (cb == ck && gk < gb) continue;
+                const bool same = gb == gk;
+                const int s1 = c->grp_start[gb], n1 = c->grp_count[gb];
+                const int s2 = c->grp_start[gk], n2 = c->grp_count[gk];
+                for (int i = 0; i < n1; ++i) {
+                    const double yp = c->pair_Y[s1 + i];
+                    int lo = 0, hi = n2;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (yp * c->pair_Y[s2 + mid] > cutoff) lo = mid + 1; else hi = mid;
+                    }
+                    for (int q = same ? i : 0; q < lo; q += chunk)
+                        T.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
+                }
+            }
+        }
+        if (c->d_gtiles[k].upload(T)) return -1;
+    }
+    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(2 * kNumQuartetClasses)) return -1;
+    c->grad_prep_cutoff = cutoff;
+    c->grad_prepared = true;
+    return 0;
 }
-extern "C" int qk_grad_host(qk_ctx* c, const double*, double*, const qk_opts*) {
-    set_err("qk_grad_host: not available yet");
-    (void)c;
-    return -5;
+
+// dgrad <- dgrad + dE2/dR for density dD (cutmatrix from dD unless given)
+static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_null, double* dgrad,
+                        const qk_opts* o, hipStream_t s) {
+    if (o->precision != QK_FP64) { set_err("the gradient is FP64 only"); return -4; }
+    if (prepare(c, o)) return -1;
+    if (grad_prepare(c, o->tau)) return -1;
+    const int n = c->nbasis, ns = c->nshell;
+    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, nullptr, n, s);
+    if (dcutmat_or_null)
+        HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
+    else
+        launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
+    launch_fill_u64(c->d_gacc.p, 3LL * c->natom, s);
+    launch_fill_u64(c->d_gcounters.p, 2LL * kNumQuartetClasses, s);
+    HIPCHK(hipGetLastError());
+    GradArgs a;
+    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
+    a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
+    a.ea = c->dk_ea.p;
+    a.eb = c->dk_eb.p;
+    a.xyz = c->d_xyz.p;
+    a.Dp = c->d_Dp.p;
+    a.cutmat = c->d_cm.p;
+    a.G = c->d_gacc.p;
+    a.n = n;
+    a.nshell = ns;
+    a.cutoff = o->tau;
+    a.hybrid = o->hybrid;
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        if (c->gtiles[k].empty()) continue;
+        ClassWork w{c->d_gtiles[k].p, (long long)c->gtiles[k].size()};
+        a.counters = c->d_gcounters.p + 2 * k;
+        const size_t lds = grad_lds_bytes(kGradBlobs[k]);
+        const int grid = (int)std::min<long long>((long long)c->gtiles[k].size(), 256LL * 16);
+        if (launch_grad_class(a, w, c->d_gtab[k].p, lds, grid, s)) {
+            set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
+            return -1;
+        }
+    }
+    launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    (void)hipSetDevice(c->device);
+    return grad_enqueue(c, dD, nullptr, dgrad, o, (hipStream_t)stream);
+}
+
+extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    (void)hipSetDevice(c->device);
+    const size_t nn = (size_t)c->nbasis * c->nbasis, m = 3 * (size_t)c->natom;
+    if (c->d_D.alloc(nn) || c->d_grad.alloc(m)) return -1;
+    HIPCHK(hipMemcpy(c->d_D.p, D, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_grad.p, grad, m * sizeof(double), hipMemcpyHostToDevice));
+    if (grad_enqueue(c, c->d_D.p, nullptr, c->d_grad.p, o, nullptr)) return -1;
+    HIPCHK(hipMemcpy(grad, c->d_grad.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// surviving quartets and primitive quartets of class k in the last gradient
+extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* np) {
+    if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipDeviceSynchronize());
+    if (!c->d_gcounters.p) { *nq = *np = 0; return 0; }
+    unsigned long long h[2];
+    HIPCHK(hipMemcpy(h, c->d_gcounters.p + 2 * k, sizeof(h), hipMemcpyDeviceToHost));
+    *nq = (long long)h[0];
+    *np = (long long)h[1];
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -896,10 +1178,33 @@ extern "C" void gpu_get2e_(double* o) {
     c->abi_have_dense = false;
     c->abi_have_cutmatrix = false;
 }
-extern "C" void gpu_upload_grad_(double*, double*) {}
-extern "C" void gpu_grad_(double*) {
-    set_err("gpu_grad: the gradient kernels are not part of this build yet");
-    fatal("gpu_grad");
+extern "C" void gpu_upload_grad_(double*, double* gradCutoff) {
+    if (!g_ctx) fatal("gpu_upload_grad (no context)");
+    g_ctx->abi_grad_cutoff = *gradCutoff;
+}
+// grad <- grad + the four-centre gradient of the uploaded density (gpu.cu:1481-1530)
+extern "C" void gpu_grad_(double* grad) {
+    qk_ctx* c = g_ctx;
+    if (!c || !c->abi_have_dense) {
+        set_err("gpu_grad: density not uploaded (call gpu_upload_calculated first)");
+        fatal("gpu_grad");
+    }
+    qk_opts op;
+    qk_default_opts(&op);
+    op.tau = c->abi_grad_cutoff;
+    op.hybrid = c->method == 0 ? 1.0 : (c->method == 1 ? 0.2 : 0.0);
+    DBuf<double> dcm;
+    const double* cmp = nullptr;
+    if (c->abi_have_cutmatrix) {
+        if (dcm.upload(c->abi_cutmatrix)) fatal("gpu_grad");
+        cmp = dcm.p;
+    }
+    (void)hipSetDevice(c->device);
+    const size_t m = 3 * (size_t)c->natom;
+    if (c->d_grad.alloc(m) || hipMemcpy(c->d_grad.p, grad, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        fatal("gpu_grad");
+    if (grad_enqueue(c, c->d_D.p, cmp, c->d_grad.p, &op, nullptr)) fatal("gpu_grad");
+    if (hipMemcpy(grad, c->d_grad.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) fatal("gpu_grad");
 }
 extern "C" void gpu_getxc_(int*, double*, double*, double*, double*, double*) {
     set_err("gpu_getxc: the XC step is not provided by this library");

@@ -10,6 +10,9 @@ namespace qk {
 
 constexpr int kNumPairClasses = 6;     // ss ps pp ds dp dd (la >= lb)
 constexpr int kNumQuartetClasses = 21;  // bra pair class >= ket pair class
+// device counters: per class (surviving quartets, primitive quartets) for the
+// FP64 kernels at [2k, 2k+1] and for the FP32 kernels at [2K+2k, 2K+2k+1]
+constexpr int kNumCounters = 4 * kNumQuartetClasses + 2;
 constexpr int kFixedShift = 48;         // Fock accumulator: int64 in units of 2^-48
 constexpr int kGradShift = 44;          // gradient accumulator: int64 in units of 2^-44
 
@@ -62,10 +65,59 @@ struct ClassWork {
     long long ntiles;
 };
 
+// FP32 copy of the kernels' primitive-pair data (same layout as DevPrims).
+struct DevPrimsF {
+    float* zeta;
+    float* oo2z;
+    float* Px;
+    float* Py;
+    float* Pz;
+    float* K;
+    float* cut;
+};
+
+// Boys table for the FP32 path: F_m(T_k), T_k = k / kBoysTabPerUnit, m < kBoysTabM
+constexpr int kBoysTabPerUnit = 16;
+constexpr int kBoysTabRows = 36 * kBoysTabPerUnit + 1;
+constexpr int kBoysTabM = 16;
+constexpr float kBoysTabTmax = 36.0f;
+
+// Nuclear-gradient build (four-centre part).  The per-class tables are one
+// int32 blob (generated, grad_tables.inc) with this header, then per stage
+// kGradStageFields ints: dir (0 A, 1 B, 2 C), source set (0 S0, 1 SA, 2 SB,
+// 3 SC), bra rows, source columns, ket rows, and the offsets of the bra HRR,
+// ket HRR and contraction tables.
+enum GradHdr {
+    kGhNodes = 0, kGhLevels, kGhSlots, kGhLT, kGhN0, kGhNA, kGhNB, kGhNC, kGhF0, kGhFA, kGhFB, kGhFC,
+    kGhStages, kGhNOut, kGhCA, kGhCB, kGhCC, kGhCD, kGhOffNodes, kGhOffLev, kGhOffTgt, kGhMaxStage,
+    kGhSize = 24
+};
+constexpr int kGradStageFields = 8;
+constexpr int kGradHrrTerms = 8;
+
+struct GradArgs {
+    DevQshells q;
+    DevPairs pr;
+    DevPrims pp;
+    const double* ea;   // exponent of the first (A or C) primitive of each primitive pair
+    const double* eb;   // exponent of the second (B or D) primitive
+    const double* xyz;
+    const double* Dp;   // cons-scaled density
+    const double* cutmat;
+    unsigned long long* G;   // 3 x natom, int64 fixed point (2^-kGradShift)
+    unsigned long long* counters;
+    int n, nshell;
+    double cutoff;      // gradient cutoff: Schwarz x density and primitive screens
+    double hybrid;
+};
+
 struct FockArgs {
     DevQshells q;
     DevPairs pr;
     DevPrims pp;    // primitive-major per group (see DevPrims)
+    DevPrimsF ppf;  // FP32 copy for the mixed-precision kernels
+    const float* Dpf;          // FP32 copy of the cons-scaled density
+    const float* boys_tab;     // kBoysTabRows x kBoysTabM
     const double* xyz;        // 3 x natom
     const double* Dp;         // cons-scaled density, n x n
     const double* cutmat;     // nshell x nshell, max |D| over shell blocks

@@ -303,6 +303,192 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 }
 
 // ---------------------------------------------------------------------------
+// FP32 path (mixed precision): quartets whose Schwarz bound Y_ij*Y_kl lies
+// below the FP32 threshold.  QUICK's Boys fit recurs upward, which is unusable
+// in single precision at small T, so this path uses a tabulated Taylor
+// expansion of F_LT about the nearest grid point (5 terms, grid 1/16) and the
+// stable downward recursion; beyond T = 36 the asymptotic form and upward
+// recursion (the exp(-T) terms are below FP32 resolution there).  The operator
+// contributions are promoted to FP64 before the fixed-point accumulation.
+template <int LT>
+__device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
+    if (T < kBoysTabTmax) {
+        const int k = __float2int_rn(T * float(kBoysTabPerUnit));
+        const float d = float(k) * (1.0f / float(kBoysTabPerUnit)) - T;
+        const float* r = tab + k * kBoysTabM + LT;
+        // F_LT(T) = sum_j F_{LT+j}(T_k) d^j / j!,  d = T_k - T, |d| <= 1/32
+        F[LT] = (((r[4] * (d * 0.25f) + r[3]) * (d * (1.0f / 3.0f)) + r[2]) * (d * 0.5f) + r[1]) * d + r[0];
+        const float e = __expf(-T);
+        const float t2 = T + T;
+#pragma unroll
+        for (int m = LT - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0f / float(2 * m + 1));
+    } else {
+        F[0] = sqrtf(0.78539816339744831f / T);
+        const float h = 0.5f / T;
+#pragma unroll
+        for (int m = 1; m <= LT; ++m) F[m] = F[m - 1] * (float(2 * m - 1) * h);
+    }
+}
+
+__device__ __forceinline__ void fx_add_f(unsigned long long* X, long long idx, float v) {
+    const long long iv = __double2ll_rn(double(v) * 0x1p48);
+    if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
+}
+
+template <int LA, int LB, int LC, int LD>
+struct OccupancyF {
+    static constexpr int nacc = nrange(LA, LA + LB) * nrange(LC, LC + LD);
+    static constexpr int value = nacc <= 36 ? 4 : 2;
+};
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock_kernel_f32(const FockArgs a, const ClassWork w) {
+    constexpr int NE = nrange(LA, LA + LB), NF = nrange(LC, LC + LD);
+    constexpr int NACC = NE * NF;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NOUT = NA * NB * NC * ND;
+    constexpr int LT = LA + LB + LC + LD;
+    const float X0 = 34.98683665524972f;
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    unsigned long long nsurv = 0, nprimq = 0;
+    const float plim = float(a.prim_limit);
+
+    for (long long t = wave0; t < w.ntiles; t += nwaves) {
+        const Tile tl = w.tiles[t];
+        const int p = __builtin_amdgcn_readfirstlane(tl.p);
+        const int q = tl.q0 + lane;
+        const bool inrange = lane < tl.len;
+        const int qa = a.pr.qa[p], qb = a.pr.qb[p];
+        const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
+        bool active = false;
+        double dn = 0.0;
+        if (inrange) {
+            const int sA = a.q.shell[qa], sB = a.q.shell[qb], sC = a.q.shell[qc], sD = a.q.shell[qd];
+            dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
+            active = a.pr.Y[q] * a.pr.Y[p] * dn > a.tau;
+        }
+        const int fa = a.q.f0[qa], fb = a.q.f0[qb];
+        const float* Dp = a.Dpf;
+        float jab[NA * NB];
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0f;
+
+        if (active) {
+            ++nsurv;
+            const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
+            const float Ax = float(a.xyz[3 * atA]), Ay = float(a.xyz[3 * atA + 1]), Az = float(a.xyz[3 * atA + 2]);
+            const float Cx = float(a.xyz[3 * atC]), Cy = float(a.xyz[3 * atC + 1]), Cz = float(a.xyz[3 * atC + 2]);
+            float acc[NACC];
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) acc[k] = 0.0f;
+            const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+            const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
+            const float dnf = float(dn);
+            for (int i = 0; i < nbp; ++i) {
+                const int bi = pb0 + i * pbs;
+                const float zab = a.ppf.zeta[bi], oo2z = a.ppf.oo2z[bi];
+                const float Px = a.ppf.Px[bi], Py = a.ppf.Py[bi], Pz = a.ppf.Pz[bi];
+                const float X2 = X0 * a.ppf.K[bi];
+                const float cp1 = dnf * a.ppf.cut[bi];
+                for (int j = 0; j < nkp; ++j) {
+                    const int kj = pk0 + j * pks;
+                    if (!(cp1 * a.ppf.cut[kj] > plim)) continue;
+                    ++nprimq;
+                    const float zcd = a.ppf.zeta[kj];
+                    const float Qx = a.ppf.Px[kj], Qy = a.ppf.Py[kj], Qz = a.ppf.Pz[kj];
+                    const float inv = 1.0f / (zab + zcd);
+                    const float roz = zcd * inv, roe = zab * inv;
+                    const float ROU = zab * roz;
+                    const float Wx = (Px * zab + Qx * zcd) * inv;
+                    const float Wy = (Py * zab + Qy * zcd) * inv;
+                    const float Wz = (Pz * zab + Qz * zcd) * inv;
+                    const float dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const float T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    float F[LT + 1];
+                    boys_f32<LT>(T, a.boys_tab, F);
+                    const float s = X2 * a.ppf.K[kj] * sqrtf(inv);
+#pragma unroll
+                    for (int m = 0; m <= LT; ++m) F[m] *= s;
+                    ClassOps<LA, LB, LC, LD>::template vrr<float>(
+                        F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
+                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.ppf.oo2z[kj], 0.5f * inv, roz, roe, acc);
+                }
+            }
+            float out[NOUT];
+            {
+                const float Bx = float(a.xyz[3 * atB]), By = float(a.xyz[3 * atB + 1]), Bz = float(a.xyz[3 * atB + 2]);
+                const float Dx = float(a.xyz[3 * atD]), Dy = float(a.xyz[3 * atD + 1]), Dz = float(a.xyz[3 * atD + 2]);
+                ClassOps<LA, LB, LC, LD>::template hrr<float>(acc, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
+                                                              Cz - Dz, out);
+            }
+            // digestion as in fock_kernel, in FP32, promoted to FP64 at the accumulate
+            const int fc = a.q.f0[qc], fd = a.q.f0[qd];
+            float wgt = 1.0f;
+            if (qa == qb) wgt *= 0.5f;
+            if (qc == qd) wgt *= 0.5f;
+            if (p == q) wgt *= 0.5f;
+            const float jw = 4.0f * wgt, kw = -float(a.hybrid) * wgt;
+#define OUT(ia, ib, ic, id) out[(((ia)*NB + (ib)) * NC + (ic)) * ND + (id)]
+            for (int ia = 0; ia < NA; ++ia)
+                for (int ib = 0; ib < NB; ++ib) {
+                    float s = 0.0f;
+                    for (int ic = 0; ic < NC; ++ic)
+                        for (int id = 0; id < ND; ++id) s += Dp[(long long)(fc + ic) * n + fd + id] * OUT(ia, ib, ic, id);
+                    jab[ia * NB + ib] = jw * s;
+                }
+            for (int ic = 0; ic < NC; ++ic)
+                for (int id = 0; id < ND; ++id) {
+                    float s = 0.0f;
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
+                    fx_add_f(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+                }
+            if (a.hybrid != 0.0) {
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int ic = 0; ic < NC; ++ic) {
+                        float s = 0.0f;
+                        for (int ib = 0; ib < NB; ++ib)
+                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
+                        fx_add_f(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
+                    }
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int id = 0; id < ND; ++id) {
+                        float s = 0.0f;
+                        for (int ib = 0; ib < NB; ++ib)
+                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
+                        fx_add_f(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
+                    }
+                for (int ib = 0; ib < NB; ++ib)
+                    for (int ic = 0; ic < NC; ++ic) {
+                        float s = 0.0f;
+                        for (int ia = 0; ia < NA; ++ia)
+                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
+                        fx_add_f(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
+                    }
+                for (int ib = 0; ib < NB; ++ib)
+                    for (int id = 0; id < ND; ++id) {
+                        float s = 0.0f;
+                        for (int ia = 0; ia < NA; ++ia)
+                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
+                        fx_add_f(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
+                    }
+            }
+#undef OUT
+        }
+#pragma unroll
+        for (int k = 0; k < NA * NB; ++k) {
+            const long long v = wave_sum_i64(__double2ll_rn(double(jab[k]) * 0x1p48));
+            if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+        }
+    }
+    if (nsurv) atomicAdd(a.counters + 0, nsurv);
+    if (nprimq) atomicAdd(a.counters + 1, nprimq);
+}
+
+// ---------------------------------------------------------------------------
 // Schwarz bounds (schwarz.f90:46-548): for every Qshell pair of one pair class,
 //   ymax[p]        = max over components of |(ab|ab)| contracted over all
 //                    primitive quartets (classcutoff), and

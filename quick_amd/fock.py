@@ -75,6 +75,9 @@ def lib() -> C.CDLL:
         L.qk_class_lane_fill.argtypes = [P, C.c_int]
         L.qk_class_stats.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                      C.POINTER(C.c_double), C.POINTER(C.c_float)]
+        L.qk_grad_class_stats.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.qk_class_stats_fp32.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -165,6 +168,35 @@ class FockEngine:
                "qk_fock_host")
         return F.reshape(self.nbasis, self.nbasis, order="F")
 
+    def grad_device(self, D_ptr: int, grad_ptr: int, opts: QkOpts | None = None, stream: int = 0):
+        """Enqueue grad <- grad + dE2/dR on ``stream`` (device pointers: D n*n, grad 3*natom)."""
+        opts = opts or make_opts()
+        _check(lib().qk_grad_device(self.h, C.c_void_p(D_ptr), C.c_void_p(grad_ptr), C.byref(opts),
+                                    C.c_void_p(stream)), "qk_grad_device")
+
+    def grad(self, D, opts: QkOpts | None = None) -> np.ndarray:
+        """Four-centre gradient dE2/dR (3 x natom) at fixed D, E2 = 1/2 sum D G(D);
+        opts.tau is the gradient cutoff."""
+        opts = opts or make_opts()
+        Df = _f(D, np.float64)
+        g = np.zeros(3 * self.natom)
+        _check(lib().qk_grad_host(self.h, _ptr(Df, C.c_double), _ptr(g, C.c_double), C.byref(opts)),
+               "qk_grad_host")
+        return g.reshape(3, self.natom, order="F")
+
+    def grad_class_stats(self):
+        """[(tag, quartets, prim_quartets)] of the last gradient."""
+        out = []
+        L = lib()
+        l4 = (C.c_int * 4)()
+        w = C.c_longlong()
+        nq, npq = C.c_longlong(), C.c_longlong()
+        for k in range(L.qk_num_classes()):
+            _check(L.qk_class_info(self.h, k, l4, C.byref(w)), "qk_class_info")
+            _check(L.qk_grad_class_stats(self.h, k, C.byref(nq), C.byref(npq)), "qk_grad_class_stats")
+            out.append(("".join("spd"[x] for x in l4), nq.value, npq.value))
+        return out
+
     def stats(self) -> QkStats:
         s = QkStats()
         _check(lib().qk_last_stats(self.h, C.byref(s)), "qk_last_stats")
@@ -174,7 +206,7 @@ class FockEngine:
         _check(lib().qk_set_timing(self.h, int(on)), "qk_set_timing")
 
     def class_stats(self):
-        """[(tag, quartets, prim_quartets, flops, ms)] for the last build."""
+        """[(tag, quartets, prim_quartets, flops, ms, lane_fill)] of the FP64 kernels in the last build."""
         out = []
         L = lib()
         l4 = (C.c_int * 4)()
@@ -186,6 +218,20 @@ class FockEngine:
                    "qk_class_stats")
             out.append(("".join("spd"[x] for x in l4), nq.value, npq.value, fl.value, ms.value,
                         L.qk_class_lane_fill(self.h, k)))
+        return out
+
+    def class_stats_fp32(self):
+        """[(tag, quartets, prim_quartets, flops, ms)] of the FP32 kernels in the last build."""
+        out = []
+        L = lib()
+        l4 = (C.c_int * 4)()
+        w = C.c_longlong()
+        nq, npq, fl, ms = C.c_longlong(), C.c_longlong(), C.c_double(), C.c_float()
+        for k in range(L.qk_num_classes()):
+            _check(L.qk_class_info(self.h, k, l4, C.byref(w)), "qk_class_info")
+            _check(L.qk_class_stats_fp32(self.h, k, C.byref(nq), C.byref(npq), C.byref(fl), C.byref(ms)),
+                   "qk_class_stats_fp32")
+            out.append(("".join("spd"[x] for x in l4), nq.value, npq.value, fl.value, ms.value))
         return out
 
     def class_work(self):

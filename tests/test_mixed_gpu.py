@@ -1,0 +1,87 @@
+"""Mixed precision (quartets with Schwarz bound Y_ij*Y_kl below the FP32
+threshold in FP32, the rest in FP64, FP64 fixed-point accumulation) against
+the all-FP64 build, which the FP64 parity tests pin to the reference."""
+import numpy as np
+import pytest
+
+from quick_amd.basis import build_basis
+from quick_amd.fock import FockEngine, make_opts, QK_FP64, QK_MIXED
+from quick_amd.geometry import CONFIGS, synthetic_density
+
+pytestmark = pytest.mark.gpu
+
+TOL_E_MIXED = 1e-6   # |dE| <= 1 uEh against FP64 at the default threshold
+
+
+def _banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+def _energy(D, G):
+    return 0.5 * float(np.sum(D * G))
+
+
+def test_threshold_zero_is_the_fp64_build(golden):
+    _banner("mixed thr=0 == fp64 water4")
+    fx = golden("water4")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G64 = eng.fock(fx.D, make_opts(precision=QK_FP64))
+    Gm = eng.fock(fx.D, make_opts(precision=QK_MIXED, fp32_threshold=0.0))
+    assert eng.stats().quartets_fp32 == 0
+    assert np.array_equal(G64, Gm)
+
+
+def test_mixed_small_system_against_reference(golden):
+    """(H2O)4 with a high threshold (most quartets in FP32) against the
+    reference's own FP64 G."""
+    _banner("mixed water4 thr=1e-2")
+    fx = golden("water4")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G = eng.fock(fx.D, make_opts(precision=QK_MIXED, fp32_threshold=1e-2))
+    st = eng.stats()
+    assert st.quartets_fp32 > st.quartets // 2
+    dF = np.abs(G - fx.G).max()
+    dE = abs(_energy(fx.D, G) - _energy(fx.D, fx.G))
+    print(f"water4 mixed thr 1e-2: fp32 quartets {st.quartets_fp32}/{st.quartets} max|dF| {dF:.2e} |dE| {dE:.2e}")
+    assert np.array_equal(G, G.T)
+    assert dF <= 1e-5
+    assert dE <= TOL_E_MIXED
+
+
+def test_mixed_is_bitwise_deterministic(golden):
+    _banner("mixed determinism water4")
+    fx = golden("water4")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    o = make_opts(precision=QK_MIXED, fp32_threshold=1e-3)
+    assert np.array_equal(eng.fock(fx.D, o), eng.fock(fx.D, o))
+
+
+def test_mixed_water32_threshold_sweep():
+    """(H2O)32 HF/6-31G**: energy and max Fock-element error of the mixed build
+    against FP64 for thresholds 1e-3 ... 1e-6."""
+    _banner("mixed water32 sweep")
+    syms, xyz = CONFIGS["water32"]()
+    b = build_basis(syms, xyz)
+    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, 7)
+    eng = FockEngine(b)
+    eng.compute_schwarz()
+    G64 = eng.fock(D, make_opts(precision=QK_FP64))
+    E64 = _energy(D, G64)
+    rows = []
+    for thr in (1e-3, 1e-4, 1e-5, 1e-6):
+        G = eng.fock(D, make_opts(precision=QK_MIXED, fp32_threshold=thr))
+        st = eng.stats()
+        dE = abs(_energy(D, G) - E64)
+        dF = np.abs(G - G64).max()
+        rows.append((thr, st.quartets_fp32 / st.quartets, dE, dF))
+        print(f"thr {thr:.0e}: fp32 share {st.quartets_fp32 / st.quartets:.3f} |dE| {dE:.2e} max|dF| {dF:.2e}",
+              flush=True)
+        assert np.array_equal(G, G.T)
+    # the default threshold (1e-5) and below meet the 1 uEh bar
+    for thr, share, dE, dF in rows:
+        if thr <= 1e-5:
+            assert dE <= TOL_E_MIXED, (thr, dE)
+            assert share > 0.0

@@ -10,7 +10,10 @@ Pins, per restated function:
   * Schwarz bounds and G(D): element-wise against the reference CPU path's own
     outputs for 5 seeded systems (tests/golden/*.npz, written by
     tests/golden/make_golden.py from oracle/_ref);
-  * a planted mistake (K with the wrong weight) is caught by the golden check.
+  * a planted mistake (K with the wrong weight) is caught by the golden check;
+  * the four-centre gradient: against the reference's hfgrad output (grad4c) for
+    the same 5 systems, translational invariance, and central finite differences
+    of the oracle's own E2 = 1/2 sum D G(D) (HF and 0.2 exchange weight).
 """
 import math
 
@@ -155,3 +158,63 @@ def test_planted_mistake_is_caught(golden):
     assert np.abs(G - fx.G).max() > 1e-4
     G, _, _ = o.fock(fx.D.T * 1.0, mode="gpu_cpuboys", pair_cut=0.0, tau=1e-3)   # wrong cutoff
     assert np.abs(G - fx.G).max() > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# four-centre gradient (gradient.f90:261-320, shell.f90:1863-2535, hrr.f90:862-1014)
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_grad_matches_reference(golden, name):
+    """Against the reference's own hfgrad four-centre loop (grad4c, gradCutoff 1e-11)."""
+    fx = golden(name)
+    o = Oracle(fx)
+    o.set_cutoffs(fx.Ycutoff, fx.cutprim)
+    g, nq, _ = o.grad(fx.D, cutoff=1e-11)
+    assert nq > 0
+    assert np.abs(g - fx.grad4c).max() <= 1e-11
+    # translational invariance: the forces on the atoms sum to zero
+    assert np.abs(g.sum(axis=1)).max() <= 1e-11
+
+
+def _e2_unscreened(fx, xyz, hybrid):
+    import copy
+    f = copy.copy(fx)
+    f.xyz = xyz
+    o = Oracle(f)
+    o.boys_exact(True)
+    Y, cp = o.schwarz()
+    o.set_cutoffs(Y, cp)
+    G, _, _ = o.fock(fx.D, tau=0.0, prim_limit=0.0, pair_cut=-1.0, hybrid=hybrid, mode="gpu_cpuboys")
+    return 0.5 * float(np.sum(fx.D * G))
+
+
+@pytest.mark.parametrize("hybrid", [1.0, 0.2])
+def test_grad_matches_central_differences(golden, hybrid):
+    """dE2/dR at fixed D against central differences (step 1e-4 bohr) of
+    E2 = 1/2 sum D G(D) from the oracle's own Fock build, with exact Boys
+    values and no screening so that E2 is smooth in R."""
+    fx = golden("h2o")
+    o = Oracle(fx)
+    o.boys_exact(True)
+    Y, cp = o.schwarz()
+    o.set_cutoffs(Y, cp)
+    g, _, _ = o.grad(fx.D, cutoff=0.0, hybrid=hybrid)
+    h = 1e-4
+    worst = 0.0
+    for a in range(fx.xyz.shape[1]):
+        for x in range(3):
+            xp, xm = fx.xyz.copy(), fx.xyz.copy()
+            xp[x, a] += h
+            xm[x, a] -= h
+            fd = (_e2_unscreened(fx, xp, hybrid) - _e2_unscreened(fx, xm, hybrid)) / (2 * h)
+            worst = max(worst, abs(g[x, a] - fd))
+    assert worst <= 2e-7, worst
+
+
+def test_grad_planted_mistake_is_caught(golden):
+    """A wrong exchange weight moves the gradient well outside the golden tolerance."""
+    fx = golden("water2")
+    o = Oracle(fx)
+    o.set_cutoffs(fx.Ycutoff, fx.cutprim)
+    g, _, _ = o.grad(fx.D, cutoff=1e-11, hybrid=0.9)
+    assert np.abs(g - fx.grad4c).max() > 1e-4
