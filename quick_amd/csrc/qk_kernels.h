@@ -72,6 +72,81 @@ __device__ __forceinline__ void boys_ref(double X, int Mb, double* F) {
     }
 }
 
+// The same fit without branches on T: the seven regions' polynomials, zero-padded
+// to 12 coefficients (highest power first), are evaluated by one Horner chain
+// over coefficients read from a per-block copy of kBoysFit; the padding makes
+// every region's chain identical to its own shorter one (fma(0, z, c) = c), and
+// F_0 = fma(v, u, w) with (z, u, w) = (X, 2X, E), (X-2, 2X, E), (X-4, 2X, E) below
+// T = 5 and (1/X, E, sqrt(pi/4X)) above.  Lanes of a wavefront with different T
+// then execute one path instead of one per region.
+constexpr int kBoysFitRegions = 7, kBoysFitTerms = 12;
+__constant__ const double kBoysFit[kBoysFitRegions * kBoysFitTerms] = {
+    0, 0, -8.36313918003957e-08, 1.21222603512827e-06, -1.15662609053481e-05, 9.25197374512647e-05,
+    -6.40994113129432e-04, 3.78787044215009e-03, -1.85185172458485e-02, 7.14285713298222e-02,
+    -1.99999999997023e-01, 3.33333333333318e-01,
+    -1.61702782425558e-10, 1.96215250865776e-09, -2.14234468198419e-08, 2.17216556336318e-07,
+    -1.98850171329371e-06, 1.62429321438911e-05, -1.16740298039895e-04, 7.24888732052332e-04,
+    -3.79490003707156e-03, 1.61723488664661e-02, -5.29428148329736e-02, 1.15702180856167e-01,
+    -2.62453564772299e-11, 3.24031041623823e-10, -3.614965656163e-09, 3.760256799971e-08,
+    -3.553558319675e-07, 3.022556449731e-06, -2.290098979647e-05, 1.526537461148e-04,
+    -8.81947375894379e-04, 4.33207949514611e-03, -1.75257821619926e-02, 5.28406320615584e-02,
+    0, 0, 0, 0, 0, 4.6897511375022e-01, -6.9955602298985e-01, 5.3689283271887e-01, -3.2883030418398e-01,
+    2.4645596956002e-01, -4.9984072848436e-01, -3.1501078774085e-06,
+    0, 0, 0, 0, 0, 0, 0, 0, -1.8784686463512e-01, 2.2991849164985e-01, -4.9893752514047e-01, -2.1916512131607e-05,
+    0, 0, 0, 0, 0, 0, 0, 0, 0, 1.9623264149430e-01, -4.9695241464490e-01, -6.0156581186481e-05,
+    0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+template <int LT>
+__device__ __forceinline__ void boys_fit(double X, int Mb, const double* __restrict__ tab, double* F) {
+    const double PIE4 = 3.1415926535897932384626433832795 / 4.0;
+    const double E = exp(-X);
+    const bool up = (X > 3.0e-5) || (X > 3.0e-7 && Mb < 4);
+    if (up) {
+        const double XINV = 1.0 / X;
+        const int r = int(X > 1.0) + int(X > 3.0) + int(X > 5.0) + int(X > 10.0) + int(X > 15.0) + int(X > 33.0);
+        const double z = r == 0 ? X : (r == 1 ? X - 2.0 : (r == 2 ? X - 4.0 : XINV));
+        const double* c = tab + kBoysFitTerms * r;
+        double v = c[0];
+#pragma unroll
+        for (int k = 1; k < kBoysFitTerms; ++k) v = fma(v, z, c[k]);
+        const double s = sqrt(PIE4 * XINV);
+        F[0] = r <= 2 ? fma(v, X + X, E) : fma(v, E, s);
+#pragma unroll
+        for (int m = 1; m <= LT; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
+    } else {
+        double v = (1.0 - X) / double(2 * Mb + 1);
+        const double twot = X * 2.0;
+        for (int m = Mb - 1; m >= LT; --m) v = (twot * v + E) / double(m * 2 + 1);
+        F[LT] = v;
+#pragma unroll
+        for (int m = LT - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) / double(m * 2 + 1);
+    }
+}
+
+// Copies primitive pairs base, base + stride, ... (n <= 64 of them) into the
+// wavefront's slab: field f of pair i at slab[f * 64 + i] (zeta, 1/2zeta, P, K,
+// cutprim).  Every lane of the wavefront takes part; the fences order the slab's
+// earlier reads before these writes and these writes before later reads (LDS
+// operations of one wavefront complete in order).
+template <class PP, typename R>
+__device__ __forceinline__ void stage_bra(const PP& pp, int base, int stride, int n, int lane, R* slab) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < n) {
+        const int k = base + lane * stride;
+        slab[lane] = pp.zeta[k];
+        slab[64 + lane] = pp.oo2z[k];
+        slab[2 * 64 + lane] = pp.Px[k];
+        slab[3 * 64 + lane] = pp.Py[k];
+        slab[4 * 64 + lane] = pp.Pz[k];
+        slab[5 * 64 + lane] = pp.K[k];
+        slab[6 * 64 + lane] = pp.cut[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void fx_add(unsigned long long* X, long long idx, double v) {
     const long long iv = __double2ll_rn(v * 0x1p48);
     if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
@@ -153,6 +228,12 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
+    __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
+    for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    __syncthreads();
+    // the current bra's primitive pairs (up to 64 at a time), one slab per wavefront
+    __shared__ double s_bra[4][7 * 64];
+    double* bra = s_bra[threadIdx.x >> 6];
 
     // one tile per wavefront and iteration: a bra pair shared by all lanes and
     // one ket pair per lane; every lane reaches the reduction below
@@ -177,6 +258,9 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
 
+        const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+        stage_bra(a.pp, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
+
         if (active) {
             ++nsurv;
             const int Mb = a.boys_qshell_order ? LT : a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd];
@@ -188,36 +272,45 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #pragma unroll
             for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
-            const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+            // kets outer (vector loads; the next ket primitive is fetched while the
+            // current one runs), bra primitives inner, read from the lanes that hold
+            // them (v_readlane: no memory access in the inner loop)
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
-            for (int i = 0; i < nbp; ++i) {
-                const int bi = pb0 + i * pbs;
-                const double zab = a.pp.zeta[bi], oo2z = a.pp.oo2z[bi];
-                const double Px = a.pp.Px[bi], Py = a.pp.Py[bi], Pz = a.pp.Pz[bi];
-                const double X2 = X0 * a.pp.K[bi];
-                const double cp1 = dn * a.pp.cut[bi];
+            for (int i0 = 0; i0 < nbp; i0 += 64) {
+                const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
+                if (i0 > 0) stage_bra(a.pp, pb0 + i0 * pbs, pbs, ni, lane, bra);
+                double nz = a.pp.zeta[pk0], no = a.pp.oo2z[pk0], nx = a.pp.Px[pk0], ny = a.pp.Py[pk0],
+                       nzz = a.pp.Pz[pk0], nK = a.pp.K[pk0], nc = a.pp.cut[pk0];
                 for (int j = 0; j < nkp; ++j) {
-                    const int kj = pk0 + j * pks;
-                    if (!(cp1 * a.pp.cut[kj] > a.prim_limit)) continue;
-                    ++nprimq;
-                    const double zcd = a.pp.zeta[kj];
-                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
-                    const double inv = 1.0 / (zab + zcd);   // the one division per primitive quartet
-                    const double roz = zcd * inv, roe = zab * inv;
-                    const double ROU = zab * roz;
-                    const double Wx = (Px * zab + Qx * zcd) * inv;
-                    const double Wy = (Py * zab + Qy * zcd) * inv;
-                    const double Wz = (Pz * zab + Qz * zcd) * inv;
-                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                    double F[LT + 1];
-                    boys_ref<LT>(T, Mb, F);
-                    const double s = X2 * a.pp.K[kj] * sqrt(inv);
+                    const double zcd = nz, oo2e = no, Qx = nx, Qy = ny, Qz = nzz, Kcd = nK, ccd = nc;
+                    if (j + 1 < nkp) {
+                        const int kn = pk0 + (j + 1) * pks;
+                        nz = a.pp.zeta[kn]; no = a.pp.oo2z[kn]; nx = a.pp.Px[kn]; ny = a.pp.Py[kn];
+                        nzz = a.pp.Pz[kn]; nK = a.pp.K[kn]; nc = a.pp.cut[kn];
+                    }
+                    const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
+                    for (int i = 0; i < ni; ++i) {
+                        if (!((dn * bra[6 * 64 + i]) * ccd > a.prim_limit)) continue;
+                        ++nprimq;
+                        const double zab = bra[i], oo2z = bra[64 + i];
+                        const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
+                        const double inv = 1.0 / (zab + zcd);   // the one division per primitive quartet
+                        const double roz = zcd * inv, roe = zab * inv;
+                        const double ROU = zab * roz;
+                        const double Wx = (Px * zab + Qx * zcd) * inv;
+                        const double Wy = (Py * zab + Qy * zcd) * inv;
+                        const double Wz = (Pz * zab + Qz * zcd) * inv;
+                        const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                        const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                        double F[LT + 1];
+                        boys_fit<LT>(T, Mb, s_boys, F);
+                        const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
 #pragma unroll
-                    for (int m = 0; m <= LT; ++m) F[m] *= s;
-                    ClassOps<LA, LB, LC, LD>::template vrr<double>(
-                        F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
-                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.pp.oo2z[kj], 0.5 * inv, roz, roe, acc);
+                        for (int m = 0; m <= LT; ++m) F[m] *= s;
+                        ClassOps<LA, LB, LC, LD>::template vrr<double>(
+                            F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx, QCy, QCz,
+                            Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5 * inv, roz, roe, acc);
+                    }
                 }
             }
 
@@ -355,6 +448,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
     const float plim = float(a.prim_limit);
+    __shared__ float s_braf[4][7 * 64];
+    float* bra = s_braf[threadIdx.x >> 6];
 
     for (long long t = wave0; t < w.ntiles; t += nwaves) {
         const Tile tl = w.tiles[t];
@@ -375,6 +470,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
         float jab[NA * NB];
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0f;
+        const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+        stage_bra(a.ppf, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
 
         if (active) {
             ++nsurv;
@@ -384,37 +481,43 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
             float acc[NACC];
 #pragma unroll
             for (int k = 0; k < NACC; ++k) acc[k] = 0.0f;
-            const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
             const float dnf = float(dn);
-            for (int i = 0; i < nbp; ++i) {
-                const int bi = pb0 + i * pbs;
-                const float zab = a.ppf.zeta[bi], oo2z = a.ppf.oo2z[bi];
-                const float Px = a.ppf.Px[bi], Py = a.ppf.Py[bi], Pz = a.ppf.Pz[bi];
-                const float X2 = X0 * a.ppf.K[bi];
-                const float cp1 = dnf * a.ppf.cut[bi];
+            for (int i0 = 0; i0 < nbp; i0 += 64) {
+                const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
+                if (i0 > 0) stage_bra(a.ppf, pb0 + i0 * pbs, pbs, ni, lane, bra);
+                float nz = a.ppf.zeta[pk0], no = a.ppf.oo2z[pk0], nx = a.ppf.Px[pk0], ny = a.ppf.Py[pk0],
+                      nzz = a.ppf.Pz[pk0], nK = a.ppf.K[pk0], nc = a.ppf.cut[pk0];
                 for (int j = 0; j < nkp; ++j) {
-                    const int kj = pk0 + j * pks;
-                    if (!(cp1 * a.ppf.cut[kj] > plim)) continue;
-                    ++nprimq;
-                    const float zcd = a.ppf.zeta[kj];
-                    const float Qx = a.ppf.Px[kj], Qy = a.ppf.Py[kj], Qz = a.ppf.Pz[kj];
-                    const float inv = 1.0f / (zab + zcd);
-                    const float roz = zcd * inv, roe = zab * inv;
-                    const float ROU = zab * roz;
-                    const float Wx = (Px * zab + Qx * zcd) * inv;
-                    const float Wy = (Py * zab + Qy * zcd) * inv;
-                    const float Wz = (Pz * zab + Qz * zcd) * inv;
-                    const float dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                    const float T = (dx * dx + dy * dy + dz * dz) * ROU;
-                    float F[LT + 1];
-                    boys_f32<LT>(T, a.boys_tab, F);
-                    const float s = X2 * a.ppf.K[kj] * sqrtf(inv);
+                    const float zcd = nz, oo2e = no, Qx = nx, Qy = ny, Qz = nzz, Kcd = nK, ccd = nc;
+                    if (j + 1 < nkp) {
+                        const int kn = pk0 + (j + 1) * pks;
+                        nz = a.ppf.zeta[kn]; no = a.ppf.oo2z[kn]; nx = a.ppf.Px[kn]; ny = a.ppf.Py[kn];
+                        nzz = a.ppf.Pz[kn]; nK = a.ppf.K[kn]; nc = a.ppf.cut[kn];
+                    }
+                    const float QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
+                    for (int i = 0; i < ni; ++i) {
+                        if (!((dnf * bra[6 * 64 + i]) * ccd > plim)) continue;
+                        ++nprimq;
+                        const float zab = bra[i], oo2z = bra[64 + i];
+                        const float Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
+                        const float inv = 1.0f / (zab + zcd);
+                        const float roz = zcd * inv, roe = zab * inv;
+                        const float ROU = zab * roz;
+                        const float Wx = (Px * zab + Qx * zcd) * inv;
+                        const float Wy = (Py * zab + Qy * zcd) * inv;
+                        const float Wz = (Pz * zab + Qz * zcd) * inv;
+                        const float dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                        const float T = (dx * dx + dy * dy + dz * dz) * ROU;
+                        float F[LT + 1];
+                        boys_f32<LT>(T, a.boys_tab, F);
+                        const float s = X0 * bra[5 * 64 + i] * Kcd * sqrtf(inv);
 #pragma unroll
-                    for (int m = 0; m <= LT; ++m) F[m] *= s;
-                    ClassOps<LA, LB, LC, LD>::template vrr<float>(
-                        F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, Qx - Cx, Qy - Cy, Qz - Cz,
-                        Wx - Qx, Wy - Qy, Wz - Qz, oo2z, a.ppf.oo2z[kj], 0.5f * inv, roz, roe, acc);
+                        for (int m = 0; m <= LT; ++m) F[m] *= s;
+                        ClassOps<LA, LB, LC, LD>::template vrr<float>(
+                            F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx, QCy, QCz,
+                            Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5f * inv, roz, roe, acc);
+                    }
                 }
             }
             float out[NOUT];

@@ -13,11 +13,13 @@ it has not been built (``python -m quick_amd.build``).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libquick_mi355x.so"
+# the in-tree build; QUICK_MI355X_LIB selects another build of the same library
+LIB_PATH = Path(os.environ.get("QUICK_MI355X_LIB", Path(__file__).resolve().parent / "libquick_mi355x.so"))
 
 QK_FP64, QK_MIXED = 0, 1
 QK_BOYS_ORDER_SHELL, QK_BOYS_ORDER_QSHELL = 0, 1
