@@ -1,23 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark: QUICK's SCF Fock-build hot path on one MI355X.
+"""Benchmark: QUICK's SCF Fock-build hot path and four-centre gradient on one MI355X.
 
 Workload: (H2O)32, HF/6-31G** (Cartesian d, 800 basis functions), seeded water
 cluster (4x4x2 grid, 3.1 A spacing, +-0.3 A jitter, seed 2), Schwarz cutoff
-tau = 1e-11 (primitive cutoff 1e-20), all-FP64 ERIs with QUICK's Boys fit.  One
-"step" is one two-electron operator build G(D) for a fixed density: the
-density (n x n) and the operator are resident in HBM; the step runs the
-density prep, every quartet class kernel and the fixed-point finalisation.
+tau = 1e-11 (primitive cutoff 1e-20).  One "step" is one two-electron operator
+build G(D) for a fixed density: the density (n x n) and the operator are
+resident in HBM; the step runs the density prep, every quartet class kernel and
+the fixed-point finalisation.  The density is a seeded synthetic one (entries
+decaying as exp(-0.25 R^2) between function centres), so that the density
+screen behaves as for a localised SCF density.
 
-The density is a seeded synthetic one (entries decaying as exp(-0.25 R^2)
-between function centres), so that the density screen behaves as for a
-localised SCF density.
-
-Reported: Fock builds/s (value), the dominant class kernel's achieved FP64
-rate against the MI355X FP64 vector peak (roofline), and the reference CPU path
-(QUICK's own Fortran, built from source into oracle/_ref) timed on a bounded
-sample of the same build on the host (cpu_baseline).
+Legs:
+  * all-FP64 builds (the reference-parity path);
+  * mixed precision: quartets with Schwarz bound Y_ij*Y_kl below a threshold in
+    FP32, the rest FP64, accumulation in FP64 fixed point; swept over
+    thresholds, each checked against the FP64 build (|dE|, max |dF|); the
+    headline value is the fastest threshold whose |dE| <= 1 uEh;
+  * the four-centre nuclear gradient dE2/dR (FP64) on (H2O)32 and on n-icosane
+    C20H42 (510 bf), seconds per gradient;
+  * the reference CPU path (QUICK's Fortran built from source into
+    oracle/_ref) timed on a bounded sample of the same build (cpu_baseline).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                       [--thresholds 1e-2,1e-3,...|none] [--copy-kernel-mib N]
 """
 from __future__ import annotations
 
@@ -88,7 +93,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--thresholds", default="1e-3,1e-4,1e-5,1e-6",
+    ap.add_argument("--thresholds", default="1e-2,1e-3,1e-4,1e-5,1e-6",
                     help="FP32 thresholds of the mixed-precision sweep ('none': FP64 only)")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), to "
@@ -194,6 +199,37 @@ def main():
     # mixed-mode bound: time at peak for the FP64 and the FP32 flops
     t_peak_ms = (total64 / (PEAK_FP64_TFLOPS * 1e12) + total32 / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
 
+    # nuclear gradient (four-centre term, FP64): the same (H2O)32 system and
+    # n-icosane C20H42 (510 bf), dE2/dR at the fixed synthetic density
+    grads = {}
+    for gname in ("water32", "icosane"):
+        if gname == "water32":
+            gb, gD, ge = b, D, eng
+        else:
+            gsyms, gxyz = CONFIGS[gname]()
+            gb = build_basis(gsyms, gxyz)
+            gD = synthetic_density(gb.xyz, gb.katom, gb.ksumtype, gb.nbasis, DENSITY_SEED)
+            ge = FockEngine(gb)
+            ge.compute_schwarz()
+        gDd = torch.from_numpy(np.ascontiguousarray(gD.ravel(order="F"))).to(dev)
+        gout = torch.zeros(3 * ge.natom, dtype=torch.float64, device=dev)
+        gopts = make_opts(precision=QK_FP64)
+        ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)       # warm-up (builds the work lists)
+        torch.cuda.synchronize()
+        nrep = max(1, min(args.steps, 5))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(nrep):
+            ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        gs = e0.elapsed_time(e1) / nrep / 1e3
+        gst = ge.grad_class_stats()
+        grads[gname] = {"seconds": gs, "atoms": int(ge.natom), "nbasis": int(gb.nbasis),
+                        "quartets": int(sum(c[1] for c in gst)), "prim_quartets": int(sum(c[2] for c in gst)),
+                        "precision": "fp64"}
+        log(f"gradient {gname}: {gs * 1e3:.2f} ms ({grads[gname]['quartets']} quartets)")
+
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic_water32.json"
     if tfile.exists():
@@ -225,7 +261,8 @@ def main():
                    "precision": precision, "fp32_threshold": thr, "geometry_seed": 2,
                    "density_seed": DENSITY_SEED, "quartets": int(st.quartets),
                    "quartets_fp32": int(st.quartets_fp32), "prim_quartets": int(st.prim_quartets),
-                   "E2_half_trDG_fp64": E64, "gradient_s": None},
+                   "E2_half_trDG_fp64": E64, "gradient_s": grads["water32"]["seconds"]},
+        "gradient": grads,
         "fp64": {"builds_per_s": 1000.0 / ms64, "ms_per_build": ms64},
         "mixed_vs_fp64_speedup": ms64 / ms,
         "mixed_sweep": sweep,

@@ -233,9 +233,233 @@ def write(path):
         open(path, "w").write(new)
 
 
+# ---------------------------------------------------------------------------
+# Straight-line gradient code for the light classes (one quartet per lane):
+#   vrg_<tag>: the raised VRR, accumulating the four contracted sets
+#              (acc0, accA, accB, accC with weights 1, wA, wB, wC);
+#   gcon_<tag>: the six derivative terms' bra/ket HRR and their contraction
+#              with the density factors K[o] into g[9] (A, B, C x xyz).
+
+LIGHT_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+
+
+def grad_sets(cls):
+    la, lb, lc, ld = cls
+    r = lambda lo, hi: erange(max(lo, 0), hi)
+    return [(r(la - 1, la + lb), r(lc - 1, lc + ld)), (r(la + 1, la + lb + 1), r(lc, lc + ld)),
+            (r(la, la + lb + 1), r(lc, lc + ld)), (r(la, la + lb), r(lc + 1, lc + ld + 1))]
+
+
+def gen_vrr_grad(cls):
+    from gen_eri import name_of
+    sets = grad_sets(cls)
+    targets = []
+    for E, F in sets:
+        targets += [(e, f, 0) for e in E for f in F]
+    order_, seen = [], set()
+
+    def visit(node):
+        if node in seen:
+            return
+        e, f, m = node
+        deps = []
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            deps += [(e, f1, m), (e, f1, m + 1)]
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                deps += [(e, f2, m), (e, f2, m + 1)]
+            if e[i] > 0:
+                deps.append((sub(e, i), f1, m + 1))
+        elif sum(e) > 0:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            deps += [(e1, f, m), (e1, f, m + 1)]
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                deps += [(e2, f, m), (e2, f, m + 1)]
+        for d in deps:
+            visit(d)
+        seen.add(node)
+        order_.append(node)
+
+    for t in targets:
+        visit(t)
+    X = "xyz"
+    lines = []
+    for node in order_:
+        e, f, m = node
+        nm = name_of(e, f, m)
+        if sum(e) == 0 and sum(f) == 0:
+            lines.append(f"    const R {nm} = F[{m}];")
+            continue
+        if sum(f) > 0:
+            i = first_dir(f)
+            f1 = sub(f, i)
+            expr = f"QC{X[i]} * {name_of(e, f1, m)} + WQ{X[i]} * {name_of(e, f1, m + 1)}"
+            if f1[i] > 0:
+                f2 = sub(f1, i)
+                expr += f" + R({f1[i]}) * oo2e * ({name_of(e, f2, m)} - roe * {name_of(e, f2, m + 1)})"
+            if e[i] > 0:
+                expr += f" + R({e[i]}) * oo2ze * {name_of(sub(e, i), f1, m + 1)}"
+        else:
+            i = first_dir(e)
+            e1 = sub(e, i)
+            expr = f"PA{X[i]} * {name_of(e1, f, m)} + WP{X[i]} * {name_of(e1, f, m + 1)}"
+            if e1[i] > 0:
+                e2 = sub(e1, i)
+                expr += f" + R({e1[i]}) * oo2z * ({name_of(e2, f, m)} - roz * {name_of(e2, f, m + 1)})"
+        lines.append(f"    const R {nm} = {expr};")
+    acc = []
+    for s_, (E, F) in enumerate(sets):
+        w = ["", "wA * ", "wB * ", "wC * "][s_]
+        k = 0
+        for e in E:
+            for f in F:
+                acc.append(f"    acc{'0ABC'[s_]}[{k}] += {w}{name_of(e, f, 0)};")
+                k += 1
+    sig = (f"template <typename R>\n__device__ __forceinline__ void vrg_{tag(cls)}(const R* __restrict__ F, "
+           "R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, "
+           "R oo2z, R oo2e, R oo2ze, R roz, R roe, R wA, R wB, R wC, "
+           "R* __restrict__ acc0, R* __restrict__ accA, R* __restrict__ accB, R* __restrict__ accC)")
+    return f"{sig}\n{{\n" + "\n".join(lines + acc) + "\n}\n", [len(E) * len(F) for E, F in sets]
+
+
+def gen_grad_contract(cls):
+    la, lb, lc, ld = cls
+    sets = grad_sets(cls)
+    X = "xyz"
+    lines = []
+    counter = [0]
+
+    def fresh(p):
+        counter[0] += 1
+        return f"{p}{counter[0]}"
+
+    def stage_values(src, l1, l2, l3, l4):
+        """names of (a b|c d) for a in order(l1), b in order(l2), c in order(l3), d in order(l4)
+        built from set src by bra then ket HRR."""
+        E, F = sets[src]
+        eidx = {e: k for k, e in enumerate(E)}
+        fidx = {f: k for k, f in enumerate(F)}
+        nF = len(F)
+        arr = f"acc{'0ABC'[src]}"
+        bra = {}
+
+        def bra_expr(a, b, fk):
+            key = (a, b, fk)
+            if key in bra:
+                return bra[key]
+            if sum(b) == 0:
+                bra[key] = f"{arr}[{eidx[a] * nF + fk}]"
+                return bra[key]
+            i = first_dir(b)
+            b1 = sub(b, i)
+            a1 = list(a)
+            a1[i] += 1
+            x1 = bra_expr(tuple(a1), b1, fk)
+            x2 = bra_expr(a, b1, fk)
+            nm = fresh("hb")
+            lines.append(f"    const R {nm} = {x1} + AB{X[i]} * {x2};")
+            bra[key] = nm
+            return nm
+
+        ket = {}
+
+        def ket_expr(a, b, c, d):
+            key = (a, b, c, d)
+            if key in ket:
+                return ket[key]
+            if sum(d) == 0:
+                ket[key] = bra_expr(a, b, fidx[c])
+                return ket[key]
+            i = first_dir(d)
+            d1 = sub(d, i)
+            c1 = list(c)
+            c1[i] += 1
+            x1 = ket_expr(a, b, tuple(c1), d1)
+            x2 = ket_expr(a, b, c, d1)
+            nm = fresh("hk")
+            lines.append(f"    const R {nm} = {x1} + CD{X[i]} * {x2};")
+            ket[key] = nm
+            return nm
+
+        return ket_expr
+
+    stages = [(0, 1, +1), (0, 0, -1), (1, 2, +1), (1, 0, -1), (2, 3, +1), (2, 0, -1)]
+    base = [la, lb, lc, ld]
+    for which, src, sign in stages:
+        if sign < 0 and base[which] == 0:
+            continue
+        shp = list(base)
+        shp[which] += sign
+        val = stage_values(src, *shp)
+        sums = {x: [] for x in range(3)}
+        o = 0
+        for a in order(la):
+            for b in order(lb):
+                for c in order(lc):
+                    for d in order(ld):
+                        v = [a, b, c, d]
+                        for x in range(3):
+                            nx = v[which][x]
+                            if sign < 0 and nx == 0:
+                                continue
+                            w = list(v[which])
+                            w[x] += sign
+                            vv = list(v)
+                            vv[which] = tuple(w)
+                            term = val(*vv)
+                            fac = "" if sign > 0 else (f"R({nx}) * " if nx > 1 else "")
+                            sums[x].append(f"{fac}K[{o}] * {term}")
+                        o += 1
+        for x in range(3):
+            if not sums[x]:
+                continue
+            op = "+=" if sign > 0 else "-="
+            lines.append(f"    g[{3 * which + x}] {op} " + " + ".join(sums[x]) + ";")
+    sig = (f"template <typename R>\n__device__ __forceinline__ void gcon_{tag(cls)}(const R* __restrict__ acc0, "
+           "const R* __restrict__ accA, const R* __restrict__ accB, const R* __restrict__ accC, "
+           "R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, const R* __restrict__ K, R* __restrict__ g)")
+    return f"{sig}\n{{\n" + "\n".join(lines) + "\n}\n"
+
+
+def write_light(path):
+    out = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.",
+           "// Straight-line gradient code of the light classes (vrg_*: raised VRR into four",
+           "// contracted sets; gcon_*: derivative HRR and density contraction).", "#pragma once", ""]
+    for cls in LIGHT_CLASSES:
+        v, sizes = gen_vrr_grad(cls)
+        out += [v, gen_grad_contract(cls)]
+        out.append(f"struct GradSets_{tag(cls)} {{ static constexpr int n0 = {sizes[0]}, nA = {sizes[1]}, "
+                   f"nB = {sizes[2]}, nC = {sizes[3]}; }};")
+    new = "\n".join(out) + "\n"
+    if not os.path.exists(path) or open(path).read() != new:
+        open(path, "w").write(new)
+
+
+def write_light_units(outdir):
+    """One translation unit per light gradient class."""
+    os.makedirs(outdir, exist_ok=True)
+    for cls in LIGHT_CLASSES:
+        t = tag(cls)
+        body = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.", '#include "../qk_grad_light.h"',
+                "namespace qk {",
+                f"void launch_grad_light_{t}(const GradArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{",
+                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), lds, s, a, w);",
+                "}", "}  // namespace qk"]
+        fn = os.path.join(outdir, f"grad_{t}.hip")
+        new = "\n".join(body) + "\n"
+        if not os.path.exists(fn) or open(fn).read() != new:
+            open(fn, "w").write(new)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1:
         write(sys.argv[1])
+        write_light(os.path.join(os.path.dirname(sys.argv[1]), "grad_light.inc"))
+        write_light_units(os.path.join(os.path.dirname(sys.argv[1]), "gen"))
     else:
         for cls in quartet_classes():
             _, info = class_blob(cls)

@@ -289,6 +289,22 @@ int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, si
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+#define QK_DECL(T) void launch_grad_light_##T(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(dsss)
+#undef QK_DECL
+using GradLightLauncher = void (*)(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
+// classes with a one-quartet-per-lane kernel (order of gen_eri.quartet_classes())
+static const GradLightLauncher kGradLight[kNumQuartetClasses] = {
+    launch_grad_light_ssss, launch_grad_light_psss, launch_grad_light_psps, launch_grad_light_ppss, nullptr,
+    nullptr, launch_grad_light_dsss, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+bool grad_class_is_light(int k) { return kGradLight[k] != nullptr; }
+int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+    const size_t lds = a.lds_grad ? sizeof(unsigned long long) * 3 * (size_t)a.natom : 0;
+    kGradLight[k](a, w, grid, lds, s);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s) {
     hipLaunchKernelGGL(grad_finalize, dim3((m + 255) / 256), dim3(256), 0, s, G, grad, m);
 }

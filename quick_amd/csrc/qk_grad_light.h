@@ -1,0 +1,214 @@
+// qk_grad_light.h -- nuclear gradient of the light quartet classes, one
+// contracted quartet per lane (the generated straight-line code of
+// gen_grad.py: vrg_* raises the VRR by one on both sides and accumulates the
+// four contracted sets, gcon_* applies the derivative HRR and contracts with
+// the density factors).  Same work tiles, screening, Boys order and
+// weighting as the table-driven kernel in qk_grad.hip; the derivative sums of
+// every quartet are converted to int64 fixed point (2^-44) and added into a
+// per-block LDS copy of the gradient, flushed once per block (or straight into
+// global memory when the gradient does not fit in LDS).
+#pragma once
+#include "qk_kernels.h"
+#include "grad_light.inc"
+
+namespace qk {
+
+template <int LA, int LB, int LC, int LD>
+struct GradLight;
+
+#define QK_GRAD_LIGHT(la, lb, lc, ld, TAG)                                                                   \
+    template <>                                                                                              \
+    struct GradLight<la, lb, lc, ld> {                                                                       \
+        using Sets = GradSets_##TAG;                                                                         \
+        template <typename R>                                                                                \
+        static __device__ __forceinline__ void vrg(const R* F, R PAx, R PAy, R PAz, R WPx, R WPy, R WPz,   \
+                                                   R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, R oo2z, R oo2e, \
+                                                   R oo2ze, R roz, R roe, R wA, R wB, R wC, R* a0, R* aA,   \
+                                                   R* aB, R* aC) {                                           \
+            vrg_##TAG<R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, oo2z, oo2e, oo2ze,  \
+                         roz, roe, wA, wB, wC, a0, aA, aB, aC);                                              \
+        }                                                                                                    \
+        template <typename R>                                                                                \
+        static __device__ __forceinline__ void gcon(const R* a0, const R* aA, const R* aB, const R* aC,     \
+                                                    R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, const R* K,   \
+                                                    R* g) {                                                  \
+            gcon_##TAG<R>(a0, aA, aB, aC, ABx, ABy, ABz, CDx, CDy, CDz, K, g);                              \
+        }                                                                                                    \
+    };
+QK_GRAD_LIGHT(0, 0, 0, 0, ssss)
+QK_GRAD_LIGHT(1, 0, 0, 0, psss)
+QK_GRAD_LIGHT(1, 0, 1, 0, psps)
+QK_GRAD_LIGHT(1, 1, 0, 0, ppss)
+QK_GRAD_LIGHT(2, 0, 0, 0, dsss)
+#undef QK_GRAD_LIGHT
+
+// primitive pairs of the bra for the gradient: the Fock slab's seven fields
+// plus the two exponents
+__device__ __forceinline__ void stage_bra_grad(const GradArgs& a, int base, int stride, int n, int lane,
+                                               double* slab) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < n) {
+        const int k = base + lane * stride;
+        slab[lane] = a.pp.zeta[k];
+        slab[64 + lane] = a.pp.oo2z[k];
+        slab[2 * 64 + lane] = a.pp.Px[k];
+        slab[3 * 64 + lane] = a.pp.Py[k];
+        slab[4 * 64 + lane] = a.pp.Pz[k];
+        slab[5 * 64 + lane] = a.pp.K[k];
+        slab[6 * 64 + lane] = a.pp.cut[k];
+        slab[7 * 64 + lane] = a.ea[k];
+        slab[8 * 64 + lane] = a.eb[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void gl_add(unsigned long long* lds, unsigned long long* G, bool use_lds, int idx,
+                                       double v) {
+    const long long iv = __double2ll_rn(v * 0x1p44);
+    if (iv == 0) return;
+    if (use_lds) atomicAdd(lds + idx, (unsigned long long)iv);
+    else atomicAdd(G + idx, (unsigned long long)iv);
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(256, 2) grad_light_kernel(const GradArgs a, const ClassWork w) {
+    using Ops = GradLight<LA, LB, LC, LD>;
+    constexpr int n0 = Ops::Sets::n0, nA = Ops::Sets::nA, nB = Ops::Sets::nB, nC = Ops::Sets::nC;
+    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    constexpr int NOUT = NA * NB * NC * ND;
+    constexpr int LT2 = LA + LB + LC + LD + 2;
+    const double X0 = 34.98683665524972;
+    const int n = a.n;
+    const int lane = threadIdx.x & 63;
+    const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    unsigned long long nsurv = 0, nprimq = 0;
+    extern __shared__ unsigned long long s_grad[];   // 3 x natom when a.lds_grad
+    __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
+    __shared__ double s_bra[4][9 * 64];
+    double* bra = s_bra[threadIdx.x >> 6];
+    const bool use_lds = a.lds_grad != 0;
+    for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    if (use_lds)
+        for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x) s_grad[k] = 0ull;
+    __syncthreads();
+
+    for (long long t = wave0; t < w.ntiles; t += nwaves) {
+        const Tile tl = w.tiles[t];
+        const int p = __builtin_amdgcn_readfirstlane(tl.p);
+        const int q = tl.q0 + lane;
+        const bool inrange = lane < tl.len;
+        const int qa = a.pr.qa[p], qb = a.pr.qb[p];
+        const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
+        const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
+        bool active = false;
+        double dn = 0.0;
+        if (inrange && !(atA == atB && atA == atC && atA == atD)) {   // one-centre quartets vanish
+            const int sA = a.q.shell[qa], sB = a.q.shell[qb], sC = a.q.shell[qc], sD = a.q.shell[qd];
+            dn = dn_max(a.cutmat, a.nshell, sA, sB, sC, sD);
+            const double ct = a.pr.Y[q] * a.pr.Y[p];
+            active = ct > a.cutoff && ct * dn > a.cutoff;
+        }
+        const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
+        stage_bra_grad(a, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
+        if (!active) continue;
+        ++nsurv;
+        const int Mb = a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd] + 2;
+        const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
+        const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+        double acc0[n0], accA[nA], accB[nB], accC[nC];
+#pragma unroll
+        for (int k = 0; k < n0; ++k) acc0[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < nA; ++k) accA[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < nB; ++k) accB[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < nC; ++k) accC[k] = 0.0;
+        const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
+        for (int i0 = 0; i0 < nbp; i0 += 64) {
+            const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
+            if (i0 > 0) stage_bra_grad(a, pb0 + i0 * pbs, pbs, ni, lane, bra);
+            for (int j = 0; j < nkp; ++j) {
+                const int kj = pk0 + j * pks;
+                const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
+                const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
+                const double Kcd = a.pp.K[kj], ccd = a.pp.cut[kj], wC = 2.0 * a.ea[kj];
+                const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
+                for (int i = 0; i < ni; ++i) {
+                    if (!((dn * bra[6 * 64 + i]) * ccd > a.cutoff)) continue;
+                    ++nprimq;
+                    const double zab = bra[i], oo2z = bra[64 + i];
+                    const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
+                    const double inv = 1.0 / (zab + zcd);
+                    const double roz = zcd * inv, roe = zab * inv;
+                    const double ROU = zab * roz;
+                    const double Wx = (Px * zab + Qx * zcd) * inv;
+                    const double Wy = (Py * zab + Qy * zcd) * inv;
+                    const double Wz = (Pz * zab + Qz * zcd) * inv;
+                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    double F[LT2 + 1];
+                    boys_fit<LT2>(T, Mb, s_boys, F);
+                    const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
+#pragma unroll
+                    for (int m = 0; m <= LT2; ++m) F[m] *= s;
+                    Ops::template vrg<double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx, QCy, QCz,
+                                              Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5 * inv, roz, roe,
+                                              2.0 * bra[7 * 64 + i], 2.0 * bra[8 * 64 + i], wC, acc0, accA, accB,
+                                              accC);
+                }
+            }
+        }
+        // density factors K = w (4 D_ab D_cd - h (D_ac D_bd + D_ad D_bc)), cons folded into D
+        const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
+        double wq = 1.0;
+        if (qa == qb) wq *= 0.5;
+        if (qc == qd) wq *= 0.5;
+        if (p == q) wq *= 0.5;
+        double K[NOUT];
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) {
+                        const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
+                        const double Dab = a.Dp[ra + fb + ib], Dcd = a.Dp[(long long)(fc + ic) * n + fd + id];
+                        const double Dac = a.Dp[ra + fc + ic], Dbd = a.Dp[rb + fd + id];
+                        const double Dad = a.Dp[ra + fd + id], Dbc = a.Dp[rb + fc + ic];
+                        K[((ia * NB + ib) * NC + ic) * ND + id] =
+                            wq * (4.0 * Dab * Dcd - a.hybrid * (Dac * Dbd + Dad * Dbc));
+                    }
+        double g[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) g[k] = 0.0;
+        {
+            const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
+            const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+            Ops::template gcon<double>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy, Cz - Dz,
+                                       K, g);
+        }
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            gl_add(s_grad, a.G, use_lds, 3 * atA + x, g[x]);
+            gl_add(s_grad, a.G, use_lds, 3 * atB + x, g[3 + x]);
+            gl_add(s_grad, a.G, use_lds, 3 * atC + x, g[6 + x]);
+            gl_add(s_grad, a.G, use_lds, 3 * atD + x, -(g[x] + g[3 + x] + g[6 + x]));
+        }
+    }
+    if (nsurv) atomicAdd(a.counters + 0, nsurv);
+    if (nprimq) atomicAdd(a.counters + 1, nprimq);
+    if (use_lds) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x)
+            if (s_grad[k] != 0ull) atomicAdd(a.G + k, s_grad[k]);
+    }
+}
+
+}  // namespace qk

@@ -31,6 +31,8 @@ void launch_scale_density(const double* D, const double* cons, double* Dp, float
 size_t grad_lds_bytes(const int* hdr);
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s);
 void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
+bool grad_class_is_light(int k);
+int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
@@ -949,7 +951,9 @@ static int grad_prepare(qk_ctx* c, double cutoff) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
-        const int chunk = LT <= 3 ? 16 : (LT <= 5 ? 4 : 1);
+        // light classes: one quartet per lane, runs of up to 64 kets; the
+        // table-driven kernel: one quartet at a time per wavefront
+        const int chunk = grad_class_is_light(k) ? 64 : (LT <= 3 ? 16 : (LT <= 5 ? 4 : 1));
         std::vector<Tile>& T = c->gtiles[k];
         T.clear();
         const size_t ng = c->grp_cls.size();
@@ -1010,10 +1014,20 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     a.nshell = ns;
     a.cutoff = o->tau;
     a.hybrid = o->hybrid;
+    a.natom = c->natom;
+    a.lds_grad = 3 * c->natom <= 8192 ? 1 : 0;   // per-block LDS copy of the gradient up to 64 KB
     for (int k = 0; k < kNumQuartetClasses; ++k) {
         if (c->gtiles[k].empty()) continue;
         ClassWork w{c->d_gtiles[k].p, (long long)c->gtiles[k].size()};
         a.counters = c->d_gcounters.p + 2 * k;
+        if (grad_class_is_light(k)) {
+            const int grid = (int)std::min<long long>(((long long)c->gtiles[k].size() + 3) / 4, 256LL * 8);
+            if (launch_grad_light(k, a, w, grid, s)) {
+                set_err("gradient kernel launch failed for class %d", k);
+                return -1;
+            }
+            continue;
+        }
         const size_t lds = grad_lds_bytes(kGradBlobs[k]);
         const int grid = (int)std::min<long long>((long long)c->gtiles[k].size(), 256LL * 16);
         if (launch_grad_class(a, w, c->d_gtab[k].p, lds, grid, s)) {

@@ -109,6 +109,8 @@ struct GradArgs {
     int n, nshell;
     double cutoff;      // gradient cutoff: Schwarz x density and primitive screens
     double hybrid;
+    int natom;
+    int lds_grad;       // light-class kernels: accumulate per block in LDS (3 x natom) first
 };
 
 struct FockArgs {
