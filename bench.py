@@ -216,6 +216,7 @@ def main():
         gopts = make_opts(precision=QK_FP64)
         ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)       # warm-up (builds the work lists)
         torch.cuda.synchronize()
+        ge.set_timing(True)
         nrep = max(1, min(args.steps, 5))
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -225,6 +226,9 @@ def main():
         torch.cuda.synchronize()
         gs = e0.elapsed_time(e1) / nrep / 1e3
         gst = ge.grad_class_stats()
+        for c in sorted(gst, key=lambda c: -c[3]):
+            if c[1]:
+                log(f"  grad class {c[0]}: {c[1]:>10d} quartets {c[2]:>11d} prim {c[3]:8.3f} ms")
         grads[gname] = {"seconds": gs, "atoms": int(ge.natom), "nbasis": int(gb.nbasis),
                         "quartets": int(sum(c[1] for c in gst)), "prim_quartets": int(sum(c[2] for c in gst)),
                         "precision": "fp64"}

@@ -164,8 +164,9 @@ int qk_set_timing(qk_ctx* c, int on);
  * primitive quartets, algorithmic flops; and the kernel's average time in ms
  * over all builds since qk_set_timing(c, 1) (0 unless timing was on). */
 int qk_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
-/* Surviving quartets and primitive quartets of class k in the last gradient. */
-int qk_grad_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets);
+/* Surviving quartets and primitive quartets of class k in the last gradient,
+ * and its kernel's average time in ms (0 unless timing is on). */
+int qk_grad_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, float* ms);
 /* The same figures for class k's FP32 kernel (QK_MIXED; zero otherwise). */
 int qk_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
 

@@ -115,8 +115,8 @@ def grad_program(sets):
 
 
 def hrr_rows(l1, l2, src_list):
-    """(a b| for a in order(l1), b in order(l2) from rows of src_list: MAX_TERMS x
-    (row, powcode, coef), padded with coef 0."""
+    """(a b| for a in order(l1), b in order(l2) from rows of src_list: MAX_TERMS
+    terms (row, powcode, coef), padded with coef 0."""
     idx = {e: k for k, e in enumerate(src_list)}
     rows = []
     for a in order(l1):
@@ -126,6 +126,22 @@ def hrr_rows(l1, l2, src_list):
             t += [(0, 0, 0)] * (MAX_TERMS - len(t))
             rows.append(t)
     return rows
+
+
+def pack_term(row, pc, coef):
+    """one HRR term in 32 bits: row (bits 0-11), (A-B)/(C-D) power code (12-17),
+    binomial coefficient (18-21); coefficient 0 marks padding."""
+    assert 0 <= row < 4096 and 0 <= pc < 64 and 0 <= coef < 16
+    return row | (pc << 12) | (coef << 18)
+
+
+def pack_con(ix, fac):
+    """contraction entry: output index (bits 0-15, 0xffff = none), factor (signed, bits 16-31)."""
+    if ix < 0:
+        return 0xffff
+    assert ix < 0xffff and -32768 <= fac < 32768
+    v = ix | ((fac & 0xffff) << 16)
+    return v - (1 << 32) if v >= (1 << 31) else v     # as a signed int32
 
 
 def class_blob(cls):
@@ -154,6 +170,9 @@ def class_blob(cls):
     body = []
 
     def put(vals):
+        # every table starts on a 16-byte boundary (read as int4 / uint4)
+        while (len(blob) + len(body)) % 4:
+            body.append(0)
         off = len(blob) + len(body)
         body.extend(vals)
         return off
@@ -168,8 +187,8 @@ def class_blob(cls):
         bra = hrr_rows(l1, l2, Es)
         ket = hrr_rows(l3, l4, Fs)
         nR1, nR2, ncol = len(bra), len(ket), len(Fs)
-        off_bra = put([v for row in bra for term in row for v in term])
-        off_ket = put([v for row in ket for term in row for v in term])
+        off_bra = put([pack_term(*term) for row in bra for term in row])
+        off_ket = put([pack_term(*term) for row in ket for term in row])
         # contraction: for each function quartet of the class and axis, the
         # element of this stage's output and its factor (+1 or -n_x)
         shp = list(base_l)
@@ -188,13 +207,14 @@ def class_blob(cls):
                             v = [a, b, c, d]
                             nx = v[which][x]
                             if sign < 0 and nx == 0:
-                                con += [-1, 0]
+                                con.append(pack_con(-1, 0))
                                 continue
                             w = list(v[which])
                             w[x] += sign
                             v[which] = tuple(w)
                             i1, i2, i3, i4 = o1[v[0]], o2[v[1]], o3[v[2]], o4[v[3]]
-                            con += [(i1 * n2 + i2) * (n3 * n4) + i3 * n4 + i4, 1 if sign > 0 else -nx]
+                            con.append(pack_con((i1 * n2 + i2) * (n3 * n4) + i3 * n4 + i4, 1 if sign > 0 else -nx))
+                        con.append(0xffff)   # pad each quartet's record to four entries
         off_con = put(con)
         assert nR1 == len(o1) * n2 and nR2 == n3 * n4
         stage_recs.append([dirn, src, nR1, ncol, nR2, off_bra, off_ket, off_con])
@@ -240,7 +260,8 @@ def write(path):
 #   gcon_<tag>: the six derivative terms' bra/ket HRR and their contraction
 #              with the density factors K[o] into g[9] (A, B, C x xyz).
 
-LIGHT_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0)]
+LIGHT_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (2, 0, 0, 0), (1, 1, 1, 0), (2, 0, 1, 0),
+                 (2, 1, 0, 0), (1, 1, 1, 1), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 1, 0), (2, 2, 0, 0)]
 
 
 def grad_sets(cls):

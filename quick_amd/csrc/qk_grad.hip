@@ -3,7 +3,7 @@
 // dE2/dR at fixed density, E2 = 1/2 sum D G(D), over the same class- and
 // bound-sorted Qshell pair lists as the Fock build.  Per surviving quartet
 // (Y_ij Y_kl > cutoff and Y_ij Y_kl DNmax > cutoff, one-centre quartets
-// skipped; gradient.f90:289-316) one wavefront:
+// skipped; gradient.f90:289-316) one block of four wavefronts:
 //   1. runs the primitive loop: QUICK's Boys fit asked for order
 //      sum(Qfinal) + 2 (shell.f90:1906-1912), the VRR raised by one on both
 //      sides as a levelled node program in LDS, and accumulates four
@@ -51,6 +51,7 @@ __device__ __forceinline__ void boys_ref_n(double X, int Mb, int nf, double* F) 
     }
 }
 
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -62,9 +63,11 @@ __device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, dou
     if (iv != 0) atomicAdd(G + idx, (unsigned long long)iv);
 }
 
-__global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+// kGradTeam threads (one or four wavefronts) work on one quartet
+template <int kGradTeam>
+__global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;          // thread of the block (kGradTeam threads share one quartet)
     const int n = a.n;
     const int nSlots = tab[kGhSlots], nLevels = tab[kGhLevels], LT2 = tab[kGhLT] + 2;
     const int n0 = tab[kGhN0], nA = tab[kGhNA], nB = tab[kGhNB], nC = tab[kGhNC];
@@ -80,6 +83,7 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
     double* Kv = S + nS;          // density factor per function quartet
     double* pw = Kv + NOUT;       // pw[axis*4+k] = AB_axis^k, pw[12+axis*4+k] = CD_axis^k
     double* Fl = pw + 24;
+    double* gred = Fl + 16;       // per-wavefront partial sums of the nine derivatives
     const double X0 = 34.98683665524972;
     unsigned long long nsurv = 0, nprimq = 0;
 
@@ -102,7 +106,7 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
             const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
             const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
             __syncthreads();                       // previous quartet's readers of S/V are done
-            for (int k = lane; k < nS; k += 64) S[k] = 0.0;
+            for (int k = lane; k < nS; k += kGradTeam) S[k] = 0.0;
             if (lane == 0) V[0] = 0.0;
 
             const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
@@ -141,13 +145,13 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
                     const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
                     const double oo2ze = 0.5 * inv;
                     __syncthreads();
-                    for (int k = lev[0] + lane; k < lev[1]; k += 64) {
+                    for (int k = lev[0] + lane; k < lev[1]; k += kGradTeam) {
                         const uint4 nd = nodes[k];
                         V[nd.x & 0xffffu] = Fl[(nd.w >> 8) & 0xffu];
                     }
                     for (int L = 1; L < nLevels; ++L) {
                         __syncthreads();
-                        for (int k = lev[L] + lane; k < lev[L + 1]; k += 64) {
+                        for (int k = lev[L] + lane; k < lev[L + 1]; k += kGradTeam) {
                             const uint4 nd = nodes[k];
                             const unsigned fl = nd.w;
                             const bool bra = (fl & 3u) == 1u;
@@ -163,7 +167,7 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
                         }
                     }
                     __syncthreads();
-                    for (int k = lane; k < nS; k += 64) {
+                    for (int k = lane; k < nS; k += kGradTeam) {
                         const double wk = k < n0 ? 1.0 : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
                         S[k] += wk * V[tgt[k]];
                     }
@@ -186,7 +190,7 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
                 if (qa == qb) wq *= 0.5;
                 if (qc == qd) wq *= 0.5;
                 if (p == q) wq *= 0.5;
-                for (int o = lane; o < NOUT; o += 64) {
+                for (int o = lane; o < NOUT; o += kGradTeam) {
                     const int id = o % CD, ic = (o / CD) % CC, ib = (o / (CD * CC)) % CB, ia = o / (CD * CC * CB);
                     const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
                     const double Dab = a.Dp[ra + fb + ib], Dcd = a.Dp[(long long)(fc + ic) * n + fd + id];
@@ -208,53 +212,67 @@ __global__ void __launch_bounds__(64) grad_kernel(const GradArgs a, const ClassW
                 double* o1 = V;
                 double* o2 = V + nR1 * nCol;
                 __syncthreads();                   // pw, Kv, previous stage's readers
-                for (int idx = lane; idx < nR1 * nCol; idx += 64) {
+                for (int idx = lane; idx < nR1 * nCol; idx += kGradTeam) {
                     const int r1 = idx / nCol, col = idx - r1 * nCol;
-                    const int* tr = bt + r1 * (3 * kGradHrrTerms);
+                    const int4 t0 = reinterpret_cast<const int4*>(bt)[2 * r1];
+                    const int4 t1 = reinterpret_cast<const int4*>(bt)[2 * r1 + 1];
+                    const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
                     double v = 0.0;
+#pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
-                        const int cf = tr[3 * u + 2];
-                        if (cf == 0) break;
-                        const int pc = tr[3 * u + 1];
+                        const int e = tt[u];
+                        const int row = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
                         v += double(cf) * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] *
-                             Ss[tr[3 * u] * nCol + col];
+                             Ss[row * nCol + col];
                     }
                     o1[idx] = v;
                 }
                 __syncthreads();
-                for (int idx = lane; idx < nR1 * nR2; idx += 64) {
+                for (int idx = lane; idx < nR1 * nR2; idx += kGradTeam) {
                     const int r1 = idx / nR2, r2 = idx - r1 * nR2;
-                    const int* tr = kt + r2 * (3 * kGradHrrTerms);
+                    const int4 t0 = reinterpret_cast<const int4*>(kt)[2 * r2];
+                    const int4 t1 = reinterpret_cast<const int4*>(kt)[2 * r2 + 1];
+                    const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
                     double v = 0.0;
+#pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
-                        const int cf = tr[3 * u + 2];
-                        if (cf == 0) break;
-                        const int pc = tr[3 * u + 1];
+                        const int e = tt[u];
+                        const int colk = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
                         v += double(cf) * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] *
-                             o1[r1 * nCol + tr[3 * u]];
+                             o1[r1 * nCol + colk];
                     }
                     o2[idx] = v;
                 }
                 __syncthreads();
-                for (int o = lane; o < NOUT; o += 64) {
+                for (int o = lane; o < NOUT; o += kGradTeam) {
                     const double K = Kv[o];
-                    const int* cr = con + 6 * o;
+                    const int4 cr = reinterpret_cast<const int4*>(con)[o];
+                    const int ce[3] = {cr.x, cr.y, cr.z};
 #pragma unroll
                     for (int x = 0; x < 3; ++x) {
-                        const int ix = cr[2 * x];
-                        if (ix >= 0) {
-                            const double v = double(cr[2 * x + 1]) * K * o2[ix];
+                        const int ix = ce[x] & 0xffff;
+                        if (ix != 0xffff) {
+                            const double v = double(ce[x] >> 16) * K * o2[ix];
                             if (dir == 0) g[x] += v; else if (dir == 1) g[3 + x] += v; else g[6 + x] += v;
                         }
                     }
                 }
             }
+            // nine sums: over each wavefront (fixed order), then over the block's
+            // wavefronts in LDS (fixed order): reproducible
 #pragma unroll
             for (int k = 0; k < 9; ++k) g[k] = wave_sum_f64(g[k]);
+            if ((lane & 63) == 0)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) gred[(lane >> 6) * 9 + k] = g[k];
+            __syncthreads();
             if (lane < 3) {
-                const double gA = lane == 0 ? g[0] : (lane == 1 ? g[1] : g[2]);
-                const double gB = lane == 0 ? g[3] : (lane == 1 ? g[4] : g[5]);
-                const double gC = lane == 0 ? g[6] : (lane == 1 ? g[7] : g[8]);
+                double gA = 0.0, gB = 0.0, gC = 0.0;
+                for (int wv = 0; wv < kGradTeam / 64; ++wv) {
+                    gA += gred[wv * 9 + lane];
+                    gB += gred[wv * 9 + 3 + lane];
+                    gC += gred[wv * 9 + 6 + lane];
+                }
                 gx_add(a.G, 3LL * atA + lane, gA);
                 gx_add(a.G, 3LL * atB + lane, gB);
                 gx_add(a.G, 3LL * atC + lane, gC);
@@ -274,30 +292,43 @@ __global__ void grad_finalize(const unsigned long long* __restrict__ G, double* 
     if (k < m) grad[k] += (double)(long long)G[k] * 0x1p-44;
 }
 
+// four wavefronts per quartet for the classes with the largest node programs
+static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1500 ? 256 : 64; }
+
 size_t grad_lds_bytes(const int* hdr) {
     const int RV = hdr[kGhSlots] > hdr[kGhMaxStage] ? hdr[kGhSlots] : hdr[kGhMaxStage];
     const int nS = hdr[kGhN0] + hdr[kGhNA] + hdr[kGhNB] + hdr[kGhNC];
-    return sizeof(double) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16);
+    return sizeof(double) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16 + 9 * 4);
 }
 
-int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s) {
+template <int TEAM>
+static int launch_team(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s) {
     if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL(grad_kernel, dim3(grid), dim3(64), lds, s, a, w, dtab);
+    hipLaunchKernelGGL(grad_kernel<TEAM>, dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// hdr: the class's table header (host copy); dtab: the same table on the device
+int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
+                      hipStream_t s) {
+    return grad_team_of(hdr) == 256 ? launch_team<256>(a, w, dtab, lds, grid, s)
+                                    : launch_team<64>(a, w, dtab, lds, grid, s);
+}
+
 #define QK_DECL(T) void launch_grad_light_##T(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
-QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(dsss)
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(dsss) QK_DECL(ppps) QK_DECL(dsps)
+QK_DECL(dpss) QK_DECL(pppp) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpps) QK_DECL(ddss)
 #undef QK_DECL
 using GradLightLauncher = void (*)(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
 // classes with a one-quartet-per-lane kernel (order of gen_eri.quartet_classes())
 static const GradLightLauncher kGradLight[kNumQuartetClasses] = {
-    launch_grad_light_ssss, launch_grad_light_psss, launch_grad_light_psps, launch_grad_light_ppss, nullptr,
-    nullptr, launch_grad_light_dsss, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    launch_grad_light_ssss, launch_grad_light_psss, launch_grad_light_psps, launch_grad_light_ppss,
+    launch_grad_light_ppps, launch_grad_light_pppp, launch_grad_light_dsss, launch_grad_light_dsps,
+    launch_grad_light_dspp, launch_grad_light_dsds, launch_grad_light_dpss, launch_grad_light_dpps, nullptr, nullptr,
+    nullptr, launch_grad_light_ddss, nullptr, nullptr, nullptr, nullptr, nullptr};
 bool grad_class_is_light(int k) { return kGradLight[k] != nullptr; }
 int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s) {
     const size_t lds = a.lds_grad ? sizeof(unsigned long long) * 3 * (size_t)a.natom : 0;

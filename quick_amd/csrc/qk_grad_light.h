@@ -40,7 +40,22 @@ QK_GRAD_LIGHT(1, 0, 0, 0, psss)
 QK_GRAD_LIGHT(1, 0, 1, 0, psps)
 QK_GRAD_LIGHT(1, 1, 0, 0, ppss)
 QK_GRAD_LIGHT(2, 0, 0, 0, dsss)
+QK_GRAD_LIGHT(1, 1, 1, 0, ppps)
+QK_GRAD_LIGHT(2, 0, 1, 0, dsps)
+QK_GRAD_LIGHT(2, 1, 0, 0, dpss)
+QK_GRAD_LIGHT(1, 1, 1, 1, pppp)
+QK_GRAD_LIGHT(2, 0, 1, 1, dspp)
+QK_GRAD_LIGHT(2, 0, 2, 0, dsds)
+QK_GRAD_LIGHT(2, 1, 1, 0, dpps)
+QK_GRAD_LIGHT(2, 2, 0, 0, ddss)
 #undef QK_GRAD_LIGHT
+
+// one wavefront per SIMD for the classes whose four sets exceed ~100 values
+template <int LA, int LB, int LC, int LD>
+struct GradLightOcc {
+    using S = typename GradLight<LA, LB, LC, LD>::Sets;
+    static constexpr int value = S::n0 + S::nA + S::nB + S::nC <= 100 ? 2 : 1;
+};
 
 // primitive pairs of the bra for the gradient: the Fock slab's seven fields
 // plus the two exponents
@@ -74,7 +89,8 @@ __device__ __forceinline__ void gl_add(unsigned long long* lds, unsigned long lo
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256, 2) grad_light_kernel(const GradArgs a, const ClassWork w) {
+__global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) grad_light_kernel(const GradArgs a,
+                                                                                                const ClassWork w) {
     using Ops = GradLight<LA, LB, LC, LD>;
     constexpr int n0 = Ops::Sets::n0, nA = Ops::Sets::nA, nB = Ops::Sets::nB, nC = Ops::Sets::nC;
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);

@@ -29,7 +29,8 @@ size_t fock_class_scratch_f32(int k);
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
 size_t grad_lds_bytes(const int* hdr);
-int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s);
+int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
+                      hipStream_t s);
 void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
 bool grad_class_is_light(int k);
 int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s);
@@ -216,8 +217,9 @@ struct qk_ctx {
     double cls_flops32[kNumQuartetClasses] = {};
     // optional per-class kernel timing (HIP events on the launch stream)
     bool timing = false;
-    // (slot k: the FP64 launch of class k; slot kNumQuartetClasses + k: its FP32 launch)
-    static constexpr int kSlots = 2 * kNumQuartetClasses;
+    // (slot k: the FP64 launch of class k; slot kNumQuartetClasses + k: its FP32
+    // launch; slot 2 kNumQuartetClasses + k: its gradient launch)
+    static constexpr int kSlots = 3 * kNumQuartetClasses;
     hipEvent_t ev_t0[kSlots] = {}, ev_t1[kSlots] = {};
     bool cls_timed[kSlots] = {};
     double cls_ms_sum[kSlots] = {};
@@ -639,6 +641,21 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     return a;
 }
 
+// folds the previous launches' kernel times into the running sums (waits for them)
+static int fold_timing(qk_ctx* c) {
+    for (int k = 0; k < qk_ctx::kSlots; ++k) {
+        if (c->cls_timed[k]) {
+            float ms = 0.0f;
+            HIPCHK(hipEventSynchronize(c->ev_t1[k]));
+            HIPCHK(hipEventElapsedTime(&ms, c->ev_t0[k], c->ev_t1[k]));
+            c->cls_ms_sum[k] += ms;
+            c->cls_ms_n[k] += 1;
+        }
+        c->cls_timed[k] = false;
+    }
+    return 0;
+}
+
 // enqueue the whole build on `s` (classes fan out to the context's streams and join back)
 static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_null, double* dG, const qk_opts* o,
                         hipStream_t s) {
@@ -654,17 +671,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);
-    // fold the previous build's kernel times into the running sums (waits for it)
-    for (int k = 0; k < qk_ctx::kSlots; ++k) {
-        if (c->cls_timed[k]) {
-            float ms = 0.0f;
-            HIPCHK(hipEventSynchronize(c->ev_t1[k]));
-            HIPCHK(hipEventElapsedTime(&ms, c->ev_t0[k], c->ev_t1[k]));
-            c->cls_ms_sum[k] += ms;
-            c->cls_ms_n[k] += 1;
-        }
-        c->cls_timed[k] = false;
-    }
+    if (fold_timing(c)) return -1;
     // heaviest classes first, round-robin over the streams
     std::vector<int> order(kNumQuartetClasses);
     std::iota(order.begin(), order.end(), 0);
@@ -991,6 +998,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     if (o->precision != QK_FP64) { set_err("the gradient is FP64 only"); return -4; }
     if (prepare(c, o)) return -1;
     if (grad_prepare(c, o->tau)) return -1;
+    if (fold_timing(c)) return -1;
     const int n = c->nbasis, ns = c->nshell;
     launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, nullptr, n, s);
     if (dcutmat_or_null)
@@ -1020,20 +1028,24 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         if (c->gtiles[k].empty()) continue;
         ClassWork w{c->d_gtiles[k].p, (long long)c->gtiles[k].size()};
         a.counters = c->d_gcounters.p + 2 * k;
+        const int slot = 2 * kNumQuartetClasses + k;
+        if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
         if (grad_class_is_light(k)) {
             const int grid = (int)std::min<long long>(((long long)c->gtiles[k].size() + 3) / 4, 256LL * 8);
             if (launch_grad_light(k, a, w, grid, s)) {
                 set_err("gradient kernel launch failed for class %d", k);
                 return -1;
             }
-            continue;
-        }
+        } else {
         const size_t lds = grad_lds_bytes(kGradBlobs[k]);
         const int grid = (int)std::min<long long>((long long)c->gtiles[k].size(), 256LL * 16);
-        if (launch_grad_class(a, w, c->d_gtab[k].p, lds, grid, s)) {
+        if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, s)) {
             set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
             return -1;
         }
+        }
+        if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
+        c->cls_timed[slot] = c->timing;
     }
     launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, s);
     HIPCHK(hipGetLastError());
@@ -1063,16 +1075,17 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
 }
 
 // surviving quartets and primitive quartets of class k in the last gradient
-extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* np) {
+extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* np, float* ms) {
     if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
     (void)hipSetDevice(c->device);
     HIPCHK(hipDeviceSynchronize());
+    *ms = 0.0f;
     if (!c->d_gcounters.p) { *nq = *np = 0; return 0; }
     unsigned long long h[2];
     HIPCHK(hipMemcpy(h, c->d_gcounters.p + 2 * k, sizeof(h), hipMemcpyDeviceToHost));
     *nq = (long long)h[0];
     *np = (long long)h[1];
-    return 0;
+    return slot_ms(c, 2 * kNumQuartetClasses + k, ms);
 }
 
 // ---------------------------------------------------------------------------

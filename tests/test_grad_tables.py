@@ -90,27 +90,28 @@ def run_blob(cls, qs, Mb, K):
     for st in range(nstages):
         dirn, src, nR1, nCol, nR2, ob, ok, oc = b[H + SF * st: H + SF * (st + 1)]
         Ss = S[setoff[src]:]
-        bt = b[ob:ob + nR1 * NT * 3].reshape(nR1, NT, 3)
-        kt = b[ok:ok + nR2 * NT * 3].reshape(nR2, NT, 3)
-        con = b[oc:oc + nout * 6].reshape(nout, 3, 2)
+        # packed entries: term = row | pow << 12 | coef << 18; contraction = index | factor << 16
+        unpack = lambda e: (int(e) & 0xfff, (int(e) >> 12) & 63, (int(e) >> 18) & 15)
+        bt = b[ob:ob + nR1 * NT].reshape(nR1, NT)
+        kt = b[ok:ok + nR2 * NT].reshape(nR2, NT)
+        con = b[oc:oc + nout * 4].reshape(nout, 4)
         o1 = np.zeros(nR1 * nCol)
         for idx in range(nR1 * nCol):
             r1, col = divmod(idx, nCol)
-            for row, pc, cf in bt[r1]:
-                if cf == 0:
-                    break
+            for e in bt[r1]:
+                row, pc, cf = unpack(e)
                 o1[idx] += cf * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] * Ss[row * nCol + col]
         o2 = np.zeros(nR1 * nR2)
         for idx in range(nR1 * nR2):
             r1, r2 = divmod(idx, nR2)
-            for colk, pc, cf in kt[r2]:
-                if cf == 0:
-                    break
+            for e in kt[r2]:
+                colk, pc, cf = unpack(e)
                 o2[idx] += cf * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] * o1[r1 * nCol + colk]
         for o in range(nout):
             for x in range(3):
-                ix, fac = con[o, x]
-                if ix >= 0:
+                e = int(con[o, x])
+                ix, fac = e & 0xffff, e >> 16
+                if ix != 0xffff:
                     g[3 * dirn + x] += fac * K.ravel()[o] * o2[ix]
     return g
 
