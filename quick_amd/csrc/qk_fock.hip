@@ -53,13 +53,17 @@ __global__ void fill_u64(unsigned long long* p, long long count) {
     if (k < count) p[k] = 0ull;
 }
 
-// G = diag(cons) (X + X^T)/2 diag(cons) in FP64
+// G = diag(cons) (X + X^T)/2 diag(cons) in FP64, X the sum of the kXCopies copies
 __global__ void fock_finalize(const unsigned long long* __restrict__ X, const double* __restrict__ cons,
                               double* __restrict__ G, int n) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)n * n) return;
     const int i = int(k / n), j = int(k % n);
-    const long long s = (long long)(X[(long long)i * n + j] + X[(long long)j * n + i]);
+    const long long nn = (long long)n * n;
+    unsigned long long u = 0;
+#pragma unroll
+    for (int c = 0; c < kXCopies; ++c) u += X[c * nn + (long long)i * n + j] + X[c * nn + (long long)j * n + i];
+    const long long s = (long long)u;
     G[k] = cons[i] * cons[j] * ((double)s * 0x1p-49);
 }
 

@@ -420,7 +420,7 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         c->d_sh_f0.upload(sh_f0) || c->d_sh_nf.upload(sh_nf))
         return -1;
     size_t nn = (size_t)nbasis * nbasis;
-    if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
+    if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
     if (build_all_pairs(c)) return -1;
     c->have_system = true;
     c->have_schwarz = false;
@@ -667,7 +667,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
-    launch_fill_u64(c->d_X.p, (long long)n * n, s);
+    launch_fill_u64(c->d_X.p, (long long)n * n * kXCopies, s);
     launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);

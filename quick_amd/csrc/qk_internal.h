@@ -13,6 +13,11 @@ constexpr int kNumQuartetClasses = 21;  // bra pair class >= ket pair class
 // device counters: per class (surviving quartets, primitive quartets) for the
 // FP64 kernels at [2k, 2k+1] and for the FP32 kernels at [2K+2k, 2K+2k+1]
 constexpr int kNumCounters = 4 * kNumQuartetClasses + 2;
+// The operator accumulator X is kept in kXCopies copies (n x n int64 each);
+// block b adds into copy b % kXCopies, and the copies are summed at the end.
+// Consecutive blocks run on different XCDs, so each copy's atomics come
+// mostly from one XCD and concurrent atomics spread over more addresses.
+constexpr int kXCopies = 8;
 constexpr int kFixedShift = 48;         // Fock accumulator: int64 in units of 2^-48
 constexpr int kGradShift = 44;          // gradient accumulator: int64 in units of 2^-44
 

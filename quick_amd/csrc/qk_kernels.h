@@ -225,6 +225,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const double X0 = 34.98683665524972;  // 2 pi^(5/2), quick_constants_module.f90:55
     const int n = a.n;
     const int lane = threadIdx.x & 63;
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % kXCopies) * n * n;
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                     double s = 0.0;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
-                    fx_add(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+                    fx_add(Xc, (long long)(fc + ic) * n + fd + id, jw * s);
                 }
             if (a.hybrid != 0.0) {
                 for (int ia = 0; ia < NA; ++ia)
@@ -355,28 +356,28 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                         double s = 0.0;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
+                        fx_add(Xc, (long long)(fa + ia) * n + fc + ic, kw * s);
                     }
                 for (int ia = 0; ia < NA; ++ia)
                     for (int id = 0; id < ND; ++id) {
                         double s = 0.0;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
+                        fx_add(Xc, (long long)(fa + ia) * n + fd + id, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int ic = 0; ic < NC; ++ic) {
                         double s = 0.0;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
+                        fx_add(Xc, (long long)(fb + ib) * n + fc + ic, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int id = 0; id < ND; ++id) {
                         double s = 0.0;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
+                        fx_add(Xc, (long long)(fb + ib) * n + fd + id, kw * s);
                     }
             }
 #undef OUT
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) {
             const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
-            if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            if (lane == 0 && v != 0) atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
         }
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);
@@ -444,6 +445,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const float X0 = 34.98683665524972f;
     const int n = a.n;
     const int lane = threadIdx.x & 63;
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % kXCopies) * n * n;
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
@@ -547,7 +549,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                     float s = 0.0f;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
-                    fx_add_f(a.X, (long long)(fc + ic) * n + fd + id, jw * s);
+                    fx_add_f(Xc, (long long)(fc + ic) * n + fd + id, jw * s);
                 }
             if (a.hybrid != 0.0) {
                 for (int ia = 0; ia < NA; ++ia)
@@ -555,28 +557,28 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add_f(a.X, (long long)(fa + ia) * n + fc + ic, kw * s);
+                        fx_add_f(Xc, (long long)(fa + ia) * n + fc + ic, kw * s);
                     }
                 for (int ia = 0; ia < NA; ++ia)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add_f(a.X, (long long)(fa + ia) * n + fd + id, kw * s);
+                        fx_add_f(Xc, (long long)(fa + ia) * n + fd + id, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int ic = 0; ic < NC; ++ic) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add_f(a.X, (long long)(fb + ib) * n + fc + ic, kw * s);
+                        fx_add_f(Xc, (long long)(fb + ib) * n + fc + ic, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add_f(a.X, (long long)(fb + ib) * n + fd + id, kw * s);
+                        fx_add_f(Xc, (long long)(fb + ib) * n + fd + id, kw * s);
                     }
             }
 #undef OUT
@@ -584,7 +586,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) {
             const long long v = wave_sum_i64(__double2ll_rn(double(jab[k]) * 0x1p48));
-            if (lane == 0 && v != 0) atomicAdd(a.X + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            if (lane == 0 && v != 0) atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
         }
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);

@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
     const double X0 = 34.98683665524972;
     const int lane = threadIdx.x;
     const int n = a.n;
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % kXCopies) * n * n;
     const uint4* nodes = TT::nodes();
     const int* lev = TT::levels();
     const unsigned short* tgt = TT::targets();
@@ -243,7 +244,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     idx = (long long)(fb + ib) * n + fd + id;
                     wt = kw;
                 }
-                fx_add(a.X, idx, wt * s);
+                fx_add(Xc, idx, wt * s);
             }
 #undef OUT
             __syncthreads();   // LDS is reused by the next quartet
