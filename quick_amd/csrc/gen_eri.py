@@ -256,8 +256,10 @@ def write_units(outdir):
         t = tag(cls)
         body = ['// Generated by quick_amd/csrc/gen_eri.py -- do not edit.', '#include "../qk_kernels.h"',
                 'namespace qk {',
-                f'void launch_fock_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
-                f'    hipLaunchKernelGGL((fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), 0, s, a, w);',
+                f'void launch_fock_{t}(const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{',
+                f'    const void* k = reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>);',
+                '    if (lds > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);',
+                f'    hipLaunchKernelGGL((fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), lds, s, a, w);',
                 '}',
                 f'size_t fock_scratch_{t}() {{',
                 '    hipFuncAttributes at{};',
@@ -265,8 +267,10 @@ def write_units(outdir):
                 '    return at.localSizeBytes;',
                 '}']
         if cls not in TEAM_CLASSES:
-            body += [f'void launch_fock_f32_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
-                     f'    hipLaunchKernelGGL((fock_kernel_f32<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), 0, s, a, w);',
+            body += [f'void launch_fock_f32_{t}(const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{',
+                     f'    const void* k = reinterpret_cast<const void*>(&fock_kernel_f32<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>);',
+                     '    if (lds > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);',
+                     f'    hipLaunchKernelGGL((fock_kernel_f32<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), lds, s, a, w);',
                      '}',
                      f'size_t fock_scratch_f32_{t}() {{',
                      '    hipFuncAttributes at{};',

@@ -70,10 +70,11 @@ __global__ void fock_finalize(const unsigned long long* __restrict__ X, const do
 // ---------------------------------------------------------------------------
 // launchers
 
-using FockLauncher = void (*)(const FockArgs&, const ClassWork&, int, hipStream_t);
+using FockLauncher = void (*)(const FockArgs&, const ClassWork&, int, size_t, hipStream_t);
+using TeamLauncher = void (*)(const FockArgs&, const ClassWork&, int, hipStream_t);
 using SchwarzLauncher = void (*)(const int*, int, const int*, const int*, const int*, const int*, const DevQshells&,
                                  const DevPrims&, const double*, const double*, double*, double*, hipStream_t);
-#define QK_DECL(T) void launch_fock_##T(const FockArgs&, const ClassWork&, int, hipStream_t);
+#define QK_DECL(T) void launch_fock_##T(const FockArgs&, const ClassWork&, int, size_t, hipStream_t);
 QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
 QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
 QK_DECL(dpdp) QK_DECL(ddss) QK_DECL(ddps) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
@@ -102,7 +103,7 @@ static const SchwarzLauncher kSchwarzLaunchers[kNumPairClasses] = {
 QK_DECL(dpdp) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
 #undef QK_DECL
 // classes evaluated by the wavefront-cooperative kernel (one quartet per wavefront)
-static const FockLauncher kTeamLaunchers[kNumQuartetClasses] = {
+static const TeamLauncher kTeamLaunchers[kNumQuartetClasses] = {
     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
     nullptr, nullptr, launch_team_dpdp, nullptr, nullptr, launch_team_ddpp, launch_team_ddds, launch_team_dddp,
     launch_team_dddd};
@@ -118,7 +119,7 @@ static const ScratchQuery kFockScratch[kNumQuartetClasses] = {
     fock_scratch_dppp, fock_scratch_dpds, fock_scratch_dpdp, fock_scratch_ddss, fock_scratch_ddps, fock_scratch_ddpp,
     fock_scratch_ddds, fock_scratch_dddp, fock_scratch_dddd};
 
-#define QK_DECL(T) void launch_fock_f32_##T(const FockArgs&, const ClassWork&, int, hipStream_t); size_t fock_scratch_f32_##T();
+#define QK_DECL(T) void launch_fock_f32_##T(const FockArgs&, const ClassWork&, int, size_t, hipStream_t); size_t fock_scratch_f32_##T();
 QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
 QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
 QK_DECL(ddss) QK_DECL(ddps)
@@ -135,13 +136,13 @@ static const ScratchQuery kFockF32Scratch[kNumQuartetClasses] = {
     fock_scratch_f32_dpss, fock_scratch_f32_dpps, fock_scratch_f32_dppp, fock_scratch_f32_dpds, nullptr,
     fock_scratch_f32_ddss, fock_scratch_f32_ddps, nullptr, nullptr, nullptr, nullptr};
 bool fock_class_has_f32(int k) { return kFockF32Launchers[k] != nullptr; }
-void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
-    kFockF32Launchers[k](a, w, grid, s);
+void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {
+    kFockF32Launchers[k](a, w, grid, lds, s);
 }
 size_t fock_class_scratch_f32(int k) { return kFockF32Scratch[k] ? kFockF32Scratch[k]() : 0; }
 
-void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
-    kFockLaunchers[k](a, w, grid, s);
+void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {
+    kFockLaunchers[k](a, w, grid, lds, s);
 }
 size_t fock_class_scratch(int k) { return kFockScratch[k](); }
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,

@@ -20,11 +20,11 @@
 #include "grad_tables.inc"
 
 namespace qk {
-void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
+void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s);
 size_t fock_class_scratch(int k);
 bool fock_class_is_team(int k);
 bool fock_class_has_f32(int k);
-void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
+void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s);
 size_t fock_class_scratch_f32(int k);
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
@@ -112,6 +112,8 @@ static const int kClassL[kNumQuartetClasses][4] = {
     {2, 1, 2, 1}, {2, 2, 0, 0}, {2, 2, 1, 0}, {2, 2, 1, 1}, {2, 2, 2, 0}, {2, 2, 2, 1}, {2, 2, 2, 2}};
 static const int kPairL[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 
+static int ncart_of(int l) { return (l + 1) * (l + 2) / 2; }
+
 static int pair_class_of(int la, int lb) {
     for (int k = 0; k < kNumPairClasses; ++k)
         if (kPairL[k][0] == la && kPairL[k][1] == lb) return k;
@@ -119,12 +121,21 @@ static int pair_class_of(int la, int lb) {
 }
 
 struct ClassHost {
-    std::vector<Tile> tiles;     // FP64 work
+    std::vector<Tile> tiles;     // FP64 work, the tiles of one bra pair consecutive
     std::vector<Tile> tiles32;   // FP32 work (mixed precision only)
+    std::vector<int2> runs, runs32;   // [first, end) tile of each bra pair
     long long total = 0;         // quartets passing the Schwarz test
     long long total32 = 0;       // of which in the FP32 lists
     DBuf<Tile> d_tiles, d_tiles32;
+    DBuf<int2> d_runs, d_runs32;
 };
+
+static void tile_runs(const std::vector<Tile>& T, std::vector<int2>& runs) {
+    runs.clear();
+    for (size_t k = 0; k < T.size(); ++k)
+        if (runs.empty() || T[runs.back().x].p != T[k].p) runs.push_back({(int)k, (int)k + 1});
+        else runs.back().y = (int)k + 1;
+}
 
 // Boys values F_m(k / kBoysTabPerUnit) for the FP32 kernels, from the series
 // F_m(T) = exp(-T) sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)) in FP64 (all terms
@@ -529,14 +540,14 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         long long total = 0, total32 = 0;
         for (size_t gb = 0; gb < groups.size(); ++gb) {
             if (groups[gb].cls != cb) continue;
-            for (size_t gk = 0; gk < groups.size(); ++gk) {
-                if (groups[gk].cls != ck) continue;
-                if (cb == ck && gk < gb) continue;
-                const bool same = (gb == gk);
-                const Group& G1 = groups[gb];
-                const Group& G2 = groups[gk];
-                for (int i = 0; i < G1.count; ++i) {
-                    const double yp = Yp[G1.start + i];
+            const Group& G1 = groups[gb];
+            for (int i = 0; i < G1.count; ++i) {
+                const double yp = Yp[G1.start + i];
+                for (size_t gk = 0; gk < groups.size(); ++gk) {
+                    if (groups[gk].cls != ck) continue;
+                    if (cb == ck && gk < gb) continue;
+                    const bool same = (gb == gk);
+                    const Group& G2 = groups[gk];
                     // kets are sorted by Y descending: first one failing Y_ij*Y_kl > tau
                     int lo = 0, hi = G2.count;
                     while (lo < hi) {
@@ -567,6 +578,8 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                 }
             }
         }
+        tile_runs(CH.tiles, CH.runs);
+        tile_runs(CH.tiles32, CH.runs32);
         CH.total = total;
         CH.total32 = total32;
     }
@@ -586,7 +599,9 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     c->pair_Y = Yp;
     c->grad_prepared = false;
     for (auto& CH : c->classes)
-        if (CH.d_tiles.upload(CH.tiles) || CH.d_tiles32.upload(CH.tiles32)) return -1;
+        if (CH.d_tiles.upload(CH.tiles) || CH.d_tiles32.upload(CH.tiles32) || CH.d_runs.upload(CH.runs) ||
+            CH.d_runs32.upload(CH.runs32))
+            return -1;
     if (mixed) {
         auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
         if (c->dkf_zeta.upload(f32(kz)) || c->dkf_oo2z.upload(f32(ko)) || c->dkf_Px.upload(f32(kx)) ||
@@ -683,38 +698,56 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     // backing (per-lane scratch x 64 x resident waves) stays under
     // kScratchBudget; the grid-stride loop over tiles covers the rest.
     constexpr double kScratchBudget = 256.0 * 1024 * 1024;
+    constexpr size_t kRowsLdsMax = 64 * 1024;   // LDS for one bra's exchange rows
     for (int k : order) {
         ClassHost& CH = c->classes[k];
         if (CH.tiles.empty() && CH.tiles32.empty()) continue;
+        // one bra pair per block with its exchange rows in LDS, when they fit
+        const int nrow = ncart_of(kClassL[k][0]) + ncart_of(kClassL[k][1]);
+        const size_t rows_lds = sizeof(unsigned long long) * (size_t)nrow * n;
+        const bool use_runs = !fock_class_is_team(k) && rows_lds <= kRowsLdsMax;
         if (!CH.tiles.empty()) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
-            ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size()};
             a.counters = c->d_counters.p + 2 * k;
             if (fock_class_is_team(k)) {
+                ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
                 // one 64-lane block per tile; up to 16 resident per CU
                 const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
                 launch_team_class(k, a, w, grid, s);
             } else {
-                const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
                 long long cap = 256LL * 16;
                 const size_t scratch = fock_class_scratch(k);
                 if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
-                launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), s);
+                if (use_runs) {
+                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_runs.p, (long long)CH.runs.size()};
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), rows_lds, s);
+                } else {
+                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
+                    const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
+                    launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), 0, s);
+                }
             }
             HIPCHK(hipGetLastError());
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
             c->cls_timed[k] = c->timing;
         }
         if (mixed && !CH.tiles32.empty()) {
-            ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size()};
             a.counters = c->d_counters.p + 2 * kNumQuartetClasses + 2 * k;
-            const long long want = ((long long)CH.tiles32.size() + 3) / 4;
             long long cap = 256LL * 16;
             const size_t scratch = fock_class_scratch_f32(k);
             if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+            ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
+            int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
+            size_t lds = 0;
+            if (use_runs) {
+                w.runs = CH.d_runs32.p;
+                w.nruns = (long long)CH.runs32.size();
+                grid = (int)std::min<long long>(w.nruns, cap);
+                lds = rows_lds;
+            }
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
-            launch_fock_f32_class(k, a, w, (int)std::min<long long>(want, cap), s);
+            launch_fock_f32_class(k, a, w, grid, lds, s);
             HIPCHK(hipGetLastError());
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
             c->cls_timed[slot] = c->timing;

@@ -68,6 +68,12 @@ struct Tile {
 struct ClassWork {
     const Tile* tiles;
     long long ntiles;
+    // optional: runs of tiles sharing one bra pair (first tile, end tile); when
+    // set, a block takes one run at a time and sums the exchange contributions
+    // to the bra's rows of X in LDS (n columns per bra function) before adding
+    // them to X once per run
+    const int2* runs;
+    long long nruns;
 };
 
 // FP32 copy of the kernels' primitive-pair data (same layout as DevPrims).

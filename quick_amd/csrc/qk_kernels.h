@@ -147,6 +147,17 @@ __device__ __forceinline__ void stage_bra(const PP& pp, int base, int stride, in
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Exchange contribution to X[row r of the bra][col]: into the block's LDS rows
+// (xr, n columns per bra function) when the block works on one bra at a time,
+// else straight into X.
+__device__ __forceinline__ void xk_add(unsigned long long* X, unsigned long long* xr, int r, long long grow, int col,
+                                       int n, double v) {
+    const long long iv = __double2ll_rn(v * 0x1p48);
+    if (iv == 0) return;
+    if (xr) atomicAdd(xr + (long long)r * n + col, (unsigned long long)iv);
+    else atomicAdd(X + grow * n + col, (unsigned long long)iv);
+}
+
 __device__ __forceinline__ void fx_add(unsigned long long* X, long long idx, double v) {
     const long long iv = __double2ll_rn(v * 0x1p48);
     if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
@@ -238,7 +249,26 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 
     // one tile per wavefront and iteration: a bra pair shared by all lanes and
     // one ket pair per lane; every lane reaches the reduction below
-    for (long long t = wave0; t < w.ntiles; t += nwaves) {
+    // exchange rows of the current bra in LDS (run mode); dynamic shared memory
+    extern __shared__ unsigned long long s_xrows[];
+    const bool runs = w.runs != nullptr;
+    unsigned long long* const xr = runs ? s_xrows : nullptr;
+    const int wib = threadIdx.x >> 6;
+    long long run = blockIdx.x;
+    for (;;) {
+    long long t0, t1, tstep;
+    int rp = 0;
+    if (runs) {
+        if (run >= w.nruns) break;
+        const int2 rg = w.runs[run];
+        t0 = rg.x + wib; t1 = rg.y; tstep = 4;
+        rp = w.tiles[rg.x].p;
+        for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
+        __syncthreads();
+    } else {
+        t0 = wave0; t1 = w.ntiles; tstep = nwaves;
+    }
+    for (long long t = t0; t < t1; t += tstep) {
         const Tile tl = w.tiles[t];
         const int p = __builtin_amdgcn_readfirstlane(tl.p);
         const int q = tl.q0 + lane;
@@ -356,28 +386,28 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                         double s = 0.0;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add(Xc, (long long)(fa + ia) * n + fc + ic, kw * s);
+                        xk_add(Xc, xr, ia, fa + ia, fc + ic, n, (kw * s));
                     }
                 for (int ia = 0; ia < NA; ++ia)
                     for (int id = 0; id < ND; ++id) {
                         double s = 0.0;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add(Xc, (long long)(fa + ia) * n + fd + id, kw * s);
+                        xk_add(Xc, xr, ia, fa + ia, fd + id, n, (kw * s));
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int ic = 0; ic < NC; ++ic) {
                         double s = 0.0;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add(Xc, (long long)(fb + ib) * n + fc + ic, kw * s);
+                        xk_add(Xc, xr, NA + ib, fb + ib, fc + ic, n, (kw * s));
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int id = 0; id < ND; ++id) {
                         double s = 0.0;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add(Xc, (long long)(fb + ib) * n + fd + id, kw * s);
+                        xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, (kw * s));
                     }
             }
 #undef OUT
@@ -389,8 +419,27 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) {
             const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
-            if (lane == 0 && v != 0) atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            if (lane == 0 && v != 0) {
+                if (xr) atomicAdd(xr + (long long)(k / NB) * n + fb + k % NB, (unsigned long long)v);
+                else atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            }
         }
+    }
+    if (!runs) break;
+    // add the run's rows to X once
+    __syncthreads();
+    {
+        const int fa0 = a.q.f0[a.pr.qa[rp]], fb0 = a.q.f0[a.pr.qb[rp]];
+        for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) {
+            const unsigned long long v = s_xrows[k];
+            if (v == 0ull) continue;
+            const int r = k / n, col = k - r * n;
+            const int grow = r < NA ? fa0 + r : fb0 + (r - NA);
+            atomicAdd(Xc + (long long)grow * n + col, v);
+        }
+    }
+    __syncthreads();
+    run += gridDim.x;
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);
     if (nprimq) atomicAdd(a.counters + 1, nprimq);
@@ -453,7 +502,26 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     __shared__ float s_braf[4][7 * 64];
     float* bra = s_braf[threadIdx.x >> 6];
 
-    for (long long t = wave0; t < w.ntiles; t += nwaves) {
+    // exchange rows of the current bra in LDS (run mode); dynamic shared memory
+    extern __shared__ unsigned long long s_xrows[];
+    const bool runs = w.runs != nullptr;
+    unsigned long long* const xr = runs ? s_xrows : nullptr;
+    const int wib = threadIdx.x >> 6;
+    long long run = blockIdx.x;
+    for (;;) {
+    long long t0, t1, tstep;
+    int rp = 0;
+    if (runs) {
+        if (run >= w.nruns) break;
+        const int2 rg = w.runs[run];
+        t0 = rg.x + wib; t1 = rg.y; tstep = 4;
+        rp = w.tiles[rg.x].p;
+        for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
+        __syncthreads();
+    } else {
+        t0 = wave0; t1 = w.ntiles; tstep = nwaves;
+    }
+    for (long long t = t0; t < t1; t += tstep) {
         const Tile tl = w.tiles[t];
         const int p = __builtin_amdgcn_readfirstlane(tl.p);
         const int q = tl.q0 + lane;
@@ -557,28 +625,28 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add_f(Xc, (long long)(fa + ia) * n + fc + ic, kw * s);
+                        xk_add(Xc, xr, ia, fa + ia, fc + ic, n, double(kw * s));
                     }
                 for (int ia = 0; ia < NA; ++ia)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add_f(Xc, (long long)(fa + ia) * n + fd + id, kw * s);
+                        xk_add(Xc, xr, ia, fa + ia, fd + id, n, double(kw * s));
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int ic = 0; ic < NC; ++ic) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        fx_add_f(Xc, (long long)(fb + ib) * n + fc + ic, kw * s);
+                        xk_add(Xc, xr, NA + ib, fb + ib, fc + ic, n, double(kw * s));
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        fx_add_f(Xc, (long long)(fb + ib) * n + fd + id, kw * s);
+                        xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, double(kw * s));
                     }
             }
 #undef OUT
@@ -586,8 +654,27 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) {
             const long long v = wave_sum_i64(__double2ll_rn(double(jab[k]) * 0x1p48));
-            if (lane == 0 && v != 0) atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            if (lane == 0 && v != 0) {
+                if (xr) atomicAdd(xr + (long long)(k / NB) * n + fb + k % NB, (unsigned long long)v);
+                else atomicAdd(Xc + (long long)(fa + k / NB) * n + fb + k % NB, (unsigned long long)v);
+            }
         }
+    }
+    if (!runs) break;
+    // add the run's rows to X once
+    __syncthreads();
+    {
+        const int fa0 = a.q.f0[a.pr.qa[rp]], fb0 = a.q.f0[a.pr.qb[rp]];
+        for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) {
+            const unsigned long long v = s_xrows[k];
+            if (v == 0ull) continue;
+            const int r = k / n, col = k - r * n;
+            const int grow = r < NA ? fa0 + r : fb0 + (r - NA);
+            atomicAdd(Xc + (long long)grow * n + col, v);
+        }
+    }
+    __syncthreads();
+    run += gridDim.x;
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);
     if (nprimq) atomicAdd(a.counters + 1, nprimq);
