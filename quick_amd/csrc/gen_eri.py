@@ -279,10 +279,13 @@ def write_units(outdir):
                      '}']
         if cls in TEAM_CLASSES:
             body = body[:2] + ['#include "../team_tables.inc"', '#include "../qk_team.h"'] + body[2:]
-            body += [f'void launch_team_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
-                     f'    hipLaunchKernelGGL((team_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}>), dim3(grid), dim3(64),',
-                     f'                       (team_lds_bytes<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}>()), s, a, w);',
-                     '}']
+            for R, sfx in (("double", ""), ("float", "_f32")):
+                body += [f'void launch_team{sfx}_{t}(const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {{',
+                         f'    constexpr unsigned lds = team_lds_bytes<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}, {R}>();',
+                         f'    const void* k = reinterpret_cast<const void*>(&team_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}, {R}>);',
+                         '    if (lds > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);',
+                         f'    hipLaunchKernelGGL((team_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, Team_{t}, {R}>), dim3(grid), dim3(64), lds, s, a, w);',
+                         '}']
         if cls[:2] == cls[2:]:
             pt = t[:2]
             body += [f'void launch_schwarz_{pt}(const int* pairs, int np, const int* qa, const int* qb, const int* prim0,',
@@ -480,9 +483,10 @@ def write_team(path):
         out.append("};")
         out.append(f"__device__ const int kTeamLevels_{t}[{len(lev)}] = {{{', '.join(map(str, lev))}}};")
         out.append(f"__device__ const unsigned short kTeamTargets_{t}[{len(tslots)}] = {{{', '.join(map(str, tslots))}}};")
-        flat = lambda tab: ", ".join("%d, %d, %d" % term for row in tab for term in row)
-        out.append(f"__device__ const int kTeamBra_{t}[{len(bra) * 12}] = {{{flat(bra)}}};")
-        out.append(f"__device__ const int kTeamKet_{t}[{len(ket) * 12}] = {{{flat(ket)}}};")
+        # one int per term: index | power code << 12 | binomial << 18 (0 = padding)
+        flat = lambda tab: ", ".join(str(e | (p3 << 12) | (c << 18)) for row in tab for (e, p3, c) in row)
+        out.append(f"__device__ const int __attribute__((aligned(16))) kTeamBra_{t}[{len(bra) * 4}] = {{{flat(bra)}}};")
+        out.append(f"__device__ const int __attribute__((aligned(16))) kTeamKet_{t}[{len(ket) * 4}] = {{{flat(ket)}}};")
         out.append(f"struct Team_{t} {{")
         out.append(f"    static constexpr int kNodes = {len(nodes)}, kLevels = {len(lev) - 1}, kSlots = {nslots};")
         out.append(f"    static constexpr int kNE = {ne}, kNF = {nf};")

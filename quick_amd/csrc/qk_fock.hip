@@ -99,7 +99,9 @@ static const FockLauncher kFockLaunchers[kNumQuartetClasses] = {
 static const SchwarzLauncher kSchwarzLaunchers[kNumPairClasses] = {
     launch_schwarz_ss, launch_schwarz_ps, launch_schwarz_pp, launch_schwarz_ds, launch_schwarz_dp, launch_schwarz_dd};
 
-#define QK_DECL(T) void launch_team_##T(const FockArgs&, const ClassWork&, int, hipStream_t);
+#define QK_DECL(T)                                                                   \
+    void launch_team_##T(const FockArgs&, const ClassWork&, int, hipStream_t); \
+    void launch_team_f32_##T(const FockArgs&, const ClassWork&, int, hipStream_t);
 QK_DECL(dpdp) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
 #undef QK_DECL
 // classes evaluated by the wavefront-cooperative kernel (one quartet per wavefront)
@@ -107,9 +109,16 @@ static const TeamLauncher kTeamLaunchers[kNumQuartetClasses] = {
     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
     nullptr, nullptr, launch_team_dpdp, nullptr, nullptr, launch_team_ddpp, launch_team_ddds, launch_team_dddp,
     launch_team_dddd};
+static const TeamLauncher kTeamF32Launchers[kNumQuartetClasses] = {
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+    nullptr, nullptr, launch_team_f32_dpdp, nullptr, nullptr, launch_team_f32_ddpp, launch_team_f32_ddds,
+    launch_team_f32_dddp, launch_team_f32_dddd};
 bool fock_class_is_team(int k) { return kTeamLaunchers[k] != nullptr; }
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
     kTeamLaunchers[k](a, w, grid, s);
+}
+void launch_team_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+    kTeamF32Launchers[k](a, w, grid, s);
 }
 
 using ScratchQuery = size_t (*)();
@@ -135,7 +144,7 @@ static const ScratchQuery kFockF32Scratch[kNumQuartetClasses] = {
     fock_scratch_f32_pppp, fock_scratch_f32_dsss, fock_scratch_f32_dsps, fock_scratch_f32_dspp, fock_scratch_f32_dsds,
     fock_scratch_f32_dpss, fock_scratch_f32_dpps, fock_scratch_f32_dppp, fock_scratch_f32_dpds, nullptr,
     fock_scratch_f32_ddss, fock_scratch_f32_ddps, nullptr, nullptr, nullptr, nullptr};
-bool fock_class_has_f32(int k) { return kFockF32Launchers[k] != nullptr; }
+bool fock_class_has_f32(int k) { return kFockF32Launchers[k] != nullptr || kTeamF32Launchers[k] != nullptr; }
 void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {
     kFockF32Launchers[k](a, w, grid, lds, s);
 }

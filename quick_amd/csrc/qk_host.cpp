@@ -27,6 +27,7 @@ bool fock_class_has_f32(int k);
 void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s);
 size_t fock_class_scratch_f32(int k);
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
+void launch_team_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
 size_t grad_lds_bytes(const int* hdr);
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
@@ -571,8 +572,8 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                     const int chunk = fock_class_is_team(k) ? 2 : 64;
                     for (int q = q_begin; q < lo64; q += chunk)
                         CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo64 - q), 0});
-                    for (int q = lo64; q < lo; q += 64)
-                        CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(64, lo - q), 0});
+                    for (int q = lo64; q < lo; q += chunk)
+                        CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
                     if (lo > q_begin) total += lo - q_begin;
                     if (lo > lo64) total32 += lo - lo64;
                 }
@@ -739,6 +740,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
             int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
             size_t lds = 0;
+            if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 16);
             if (use_runs) {
                 w.runs = CH.d_runs32.p;
                 w.nruns = (long long)CH.runs32.size();
@@ -747,7 +749,8 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             }
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
-            launch_fock_f32_class(k, a, w, grid, lds, s);
+            if (fock_class_is_team(k)) launch_team_f32_class(k, a, w, grid, s);
+            else launch_fock_f32_class(k, a, w, grid, lds, s);
             HIPCHK(hipGetLastError());
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
             c->cls_timed[slot] = c->timing;

@@ -35,31 +35,64 @@ struct TeamLayout {
     static constexpr int total = oF + 16;
 };
 
-template <int LA, int LB, int LC, int LD, class TT>
+template <int LA, int LB, int LC, int LD, class TT, typename R = double>
 constexpr unsigned team_lds_bytes() {
-    return sizeof(double) * TeamLayout<LA, LB, LC, LD, TT>::total;
+    return sizeof(R) * TeamLayout<LA, LB, LC, LD, TT>::total;
 }
 
-template <int LA, int LB, int LC, int LD, class TT>
+// The precision-dependent inputs of the cooperative kernel: primitive-pair
+// data, density and Boys evaluation (QUICK's fit in FP64; the tabulated Taylor
+// expansion in FP32, the mixed-precision path).
+template <typename R>
+struct TeamIn;
+template <>
+struct TeamIn<double> {
+    const double *zeta, *oo2z, *Px, *Py, *Pz, *K, *cut, *Dp;
+    __device__ explicit TeamIn(const FockArgs& a)
+        : zeta(a.pp.zeta), oo2z(a.pp.oo2z), Px(a.pp.Px), Py(a.pp.Py), Pz(a.pp.Pz), K(a.pp.K), cut(a.pp.cut), Dp(a.Dp) {}
+    template <int LT>
+    __device__ __forceinline__ void boys(double T, int Mb, const double* fit, const FockArgs&, double* F) const {
+        boys_fit<LT>(T, Mb, fit, F);
+    }
+};
+template <>
+struct TeamIn<float> {
+    const float *zeta, *oo2z, *Px, *Py, *Pz, *K, *cut, *Dp;
+    __device__ explicit TeamIn(const FockArgs& a)
+        : zeta(a.ppf.zeta), oo2z(a.ppf.oo2z), Px(a.ppf.Px), Py(a.ppf.Py), Pz(a.ppf.Pz), K(a.ppf.K), cut(a.ppf.cut),
+          Dp(a.Dpf) {}
+    template <int LT>
+    __device__ __forceinline__ void boys(float T, int, const double*, const FockArgs& a, float* F) const {
+        boys_f32<LT>(T, a.boys_tab, F);
+    }
+};
+
+template <int LA, int LB, int LC, int LD, class TT, typename R>
 __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassWork w) {
     using Lo = TeamLayout<LA, LB, LC, LD, TT>;
     constexpr int NA = Lo::NA, NB = Lo::NB, NC = Lo::NC, ND = Lo::ND;
     constexpr int NAB = Lo::NAB, NCD = Lo::NCD, NOUT = Lo::NOUT, NACC = Lo::NACC, LT = Lo::LT;
     constexpr int NF = TT::kNF;
-    extern __shared__ double lds[];
-    double* V = lds;
-    double* hb = lds;                  // reuses V after the primitive loop
-    double* outv = lds + NAB * NF;
-    double* acc = lds + Lo::oAcc;
-    double* Dcd = lds + Lo::oD;
-    double* Dab = Dcd + NCD;
-    double* Dbd = Dab + NAB;
-    double* Dbc = Dbd + NB * ND;
-    double* Dad = Dbc + NB * NC;
-    double* Dac = Dad + NA * ND;
-    double* pw = lds + Lo::oPw;        // pw[axis*3 + k] = AB_axis^k, pw[9 + axis*3 + k] = CD_axis^k
-    double* Fl = lds + Lo::oF;
-    const double X0 = 34.98683665524972;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    R* lds = reinterpret_cast<R*>(lds_raw);
+    R* V = lds;
+    R* hb = lds;                  // reuses V after the primitive loop
+    R* outv = lds + NAB * NF;
+    R* acc = lds + Lo::oAcc;
+    R* Dcd = lds + Lo::oD;
+    R* Dab = Dcd + NCD;
+    R* Dbd = Dab + NAB;
+    R* Dbc = Dbd + NB * ND;
+    R* Dad = Dbc + NB * NC;
+    R* Dac = Dad + NA * ND;
+    R* pw = lds + Lo::oPw;        // pw[axis*3 + k] = AB_axis^k, pw[9 + axis*3 + k] = CD_axis^k
+    R* Fl = lds + Lo::oF;
+    __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
+    for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    __syncthreads();
+    const TeamIn<R> in(a);
+    const R X0 = R(34.98683665524972);
+    const R plim = R(a.prim_limit);
     const int lane = threadIdx.x;
     const int n = a.n;
     unsigned long long* const Xc = a.X + (long long)(blockIdx.x % kXCopies) * n * n;
@@ -82,44 +115,45 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
             ++nsurv;
             const int Mb = a.boys_qshell_order ? LT : a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd];
             const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];
-            const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
-            const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
-            for (int k = lane; k < NACC; k += 64) acc[k] = 0.0;
-            if (lane == 0) V[0] = 0.0;
+            const R Ax = R(a.xyz[3 * atA]), Ay = R(a.xyz[3 * atA + 1]), Az = R(a.xyz[3 * atA + 2]);
+            const R Cx = R(a.xyz[3 * atC]), Cy = R(a.xyz[3 * atC + 1]), Cz = R(a.xyz[3 * atC + 2]);
+            for (int k = lane; k < NACC; k += 64) acc[k] = R(0);
+            if (lane == 0) V[0] = R(0);
 
             const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
+            const R dnr = R(dn);
             for (int i = 0; i < nbp; ++i) {
                 const int bi = pb0 + i * pbs;
-                const double zab = a.pp.zeta[bi], oo2z = a.pp.oo2z[bi];
-                const double Px = a.pp.Px[bi], Py = a.pp.Py[bi], Pz = a.pp.Pz[bi];
-                const double X2 = X0 * a.pp.K[bi];
-                const double cp1 = dn * a.pp.cut[bi];
+                const R zab = in.zeta[bi], oo2z = in.oo2z[bi];
+                const R Px = in.Px[bi], Py = in.Py[bi], Pz = in.Pz[bi];
+                const R X2 = X0 * in.K[bi];
+                const R cp1 = dnr * in.cut[bi];
                 for (int j = 0; j < nkp; ++j) {
                     const int kj = pk0 + j * pks;
-                    if (!(cp1 * a.pp.cut[kj] > a.prim_limit)) continue;
+                    if (!(cp1 * in.cut[kj] > plim)) continue;
                     ++nprimq;
-                    const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
-                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
-                    const double inv = 1.0 / (zab + zcd);
-                    const double roz = zcd * inv, roe = zab * inv;
-                    const double ROU = zab * roz;
-                    const double Wx = (Px * zab + Qx * zcd) * inv;
-                    const double Wy = (Py * zab + Qy * zcd) * inv;
-                    const double Wz = (Pz * zab + Qz * zcd) * inv;
-                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                    double F[LT + 1];
-                    boys_ref<LT>(T, Mb, F);
-                    const double s = X2 * a.pp.K[kj] * sqrt(inv);
-                    double fsel = F[0] * s;
+                    const R zcd = in.zeta[kj], oo2e = in.oo2z[kj];
+                    const R Qx = in.Px[kj], Qy = in.Py[kj], Qz = in.Pz[kj];
+                    const R inv = R(1) / (zab + zcd);
+                    const R roz = zcd * inv, roe = zab * inv;
+                    const R ROU = zab * roz;
+                    const R Wx = (Px * zab + Qx * zcd) * inv;
+                    const R Wy = (Py * zab + Qy * zcd) * inv;
+                    const R Wz = (Pz * zab + Qz * zcd) * inv;
+                    const R dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const R T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    R F[LT + 1];
+                    in.template boys<LT>(T, Mb, s_boys, a, F);
+                    const R s = X2 * in.K[kj] * sqrt(inv);
+                    R fsel = F[0] * s;
 #pragma unroll
                     for (int m = 1; m <= LT; ++m) fsel = (lane == m) ? F[m] * s : fsel;
                     __syncthreads();                       // previous readers of V and Fl are done
                     if (lane <= LT) Fl[lane] = fsel;
-                    const double PA[3] = {Px - Ax, Py - Ay, Pz - Az}, WP[3] = {Wx - Px, Wy - Py, Wz - Pz};
-                    const double QC[3] = {Qx - Cx, Qy - Cy, Qz - Cz}, WQ[3] = {Wx - Qx, Wy - Qy, Wz - Qz};
-                    const double oo2ze = 0.5 * inv;
+                    const R PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
+                    const R QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
+                    const R oo2ze = R(0.5) * inv;
                     __syncthreads();
                     for (int k = lev[0] + lane; k < lev[1]; k += 64) {
                         const uint4 nd = nodes[k];
@@ -132,13 +166,13 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                             const unsigned fl = nd.w;
                             const bool bra = (fl & 3u) == 1u;
                             const int dir = (fl >> 2) & 3;
-                            const double n1 = double((fl >> 4) & 15u), ne = double((fl >> 8) & 15u);
-                            const double pa = dir == 0 ? (bra ? PA[0] : QC[0]) : (dir == 1 ? (bra ? PA[1] : QC[1]) : (bra ? PA[2] : QC[2]));
-                            const double wp = dir == 0 ? (bra ? WP[0] : WQ[0]) : (dir == 1 ? (bra ? WP[1] : WQ[1]) : (bra ? WP[2] : WQ[2]));
-                            const double c3 = n1 * (bra ? oo2z : oo2e);
-                            const double c4 = bra ? roz : roe;
-                            const double v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
-                                             c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
+                            const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
+                            const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
+                            const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
+                            const R c3 = n1 * (bra ? oo2z : oo2e);
+                            const R c4 = bra ? roz : roe;
+                            const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
+                                        c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
                             V[nd.x & 0xffffu] = v;
                         }
                     }
@@ -150,15 +184,15 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
 
             // HRR: powers of A-B and C-D, density blocks
             if (lane < 18) {
-                const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
-                const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+                const R Bx = R(a.xyz[3 * atB]), By = R(a.xyz[3 * atB + 1]), Bz = R(a.xyz[3 * atB + 2]);
+                const R Dx = R(a.xyz[3 * atD]), Dy = R(a.xyz[3 * atD + 1]), Dz = R(a.xyz[3 * atD + 2]);
                 const int grp = lane / 9, axis = (lane % 9) / 3, k = lane % 3;
-                const double d = grp == 0 ? (axis == 0 ? Ax - Bx : (axis == 1 ? Ay - By : Az - Bz))
-                                          : (axis == 0 ? Cx - Dx : (axis == 1 ? Cy - Dy : Cz - Dz));
-                pw[lane] = k == 0 ? 1.0 : (k == 1 ? d : d * d);
+                const R d = grp == 0 ? (axis == 0 ? Ax - Bx : (axis == 1 ? Ay - By : Az - Bz))
+                                     : (axis == 0 ? Cx - Dx : (axis == 1 ? Cy - Dy : Cz - Dz));
+                pw[lane] = k == 0 ? R(1) : (k == 1 ? d : d * d);
             }
             const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
-            const double* Dp = a.Dp;
+            const auto* Dp = in.Dp;
             for (int k = lane; k < NCD; k += 64) Dcd[k] = Dp[(long long)(fc + k / ND) * n + fd + k % ND];
             for (int k = lane; k < NAB; k += 64) Dab[k] = Dp[(long long)(fa + k / NB) * n + fb + k % NB];
             for (int k = lane; k < NB * ND; k += 64) Dbd[k] = Dp[(long long)(fb + k / ND) * n + fd + k % ND];
@@ -166,27 +200,32 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
             for (int k = lane; k < NA * ND; k += 64) Dad[k] = Dp[(long long)(fa + k / ND) * n + fd + k % ND];
             for (int k = lane; k < NA * NC; k += 64) Dac[k] = Dp[(long long)(fa + k / NC) * n + fc + k % NC];
             __syncthreads();
-            const int* bt = TT::bra();
+            // HRR tables: per output row four terms packed as (index | power code << 12 | binomial << 18)
+            const int4* bt = reinterpret_cast<const int4*>(TT::bra());
             for (int h = lane; h < NAB * NF; h += 64) {
                 const int ab = h / NF, f = h % NF;
-                double v = 0.0;
+                const int4 row = bt[ab];
+                const int tt[4] = {row.x, row.y, row.z, row.w};
+                R v = R(0);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int e = bt[ab * 12 + 3 * u], pw3 = bt[ab * 12 + 3 * u + 1], cf = bt[ab * 12 + 3 * u + 2];
-                    const double c = double(cf) * pw[pw3 & 3] * pw[3 + ((pw3 >> 2) & 3)] * pw[6 + ((pw3 >> 4) & 3)];
+                    const int e = tt[u] & 0xfff, pw3 = (tt[u] >> 12) & 63, cf = (tt[u] >> 18) & 15;
+                    const R c = R(cf) * pw[pw3 & 3] * pw[3 + ((pw3 >> 2) & 3)] * pw[6 + ((pw3 >> 4) & 3)];
                     v += c * acc[e * NF + f];
                 }
                 hb[h] = v;
             }
             __syncthreads();
-            const int* kt = TT::ket();
+            const int4* kt = reinterpret_cast<const int4*>(TT::ket());
             for (int o = lane; o < NOUT; o += 64) {
                 const int ab = o / NCD, cd = o % NCD;
-                double v = 0.0;
+                const int4 row = kt[cd];
+                const int tt[4] = {row.x, row.y, row.z, row.w};
+                R v = R(0);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int f = kt[cd * 12 + 3 * u], pw3 = kt[cd * 12 + 3 * u + 1], cf = kt[cd * 12 + 3 * u + 2];
-                    const double c = double(cf) * pw[9 + (pw3 & 3)] * pw[12 + ((pw3 >> 2) & 3)] * pw[15 + ((pw3 >> 4) & 3)];
+                    const int f = tt[u] & 0xfff, pw3 = (tt[u] >> 12) & 63, cf = (tt[u] >> 18) & 15;
+                    const R c = R(cf) * pw[9 + (pw3 & 3)] * pw[12 + ((pw3 >> 2) & 3)] * pw[15 + ((pw3 >> 4) & 3)];
                     v += c * hb[ab * NF + f];
                 }
                 outv[o] = v;
@@ -194,19 +233,19 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
             __syncthreads();
 
             // digestion (see fock_kernel): G = (X + X^T)/2
-            double wgt = 1.0;
-            if (qa == qb) wgt *= 0.5;
-            if (qc == qd) wgt *= 0.5;
-            if (p == q) wgt *= 0.5;
-            const double jw = 4.0 * wgt, kw = -a.hybrid * wgt;
+            R wgt = R(1);
+            if (qa == qb) wgt *= R(0.5);
+            if (qc == qd) wgt *= R(0.5);
+            if (p == q) wgt *= R(0.5);
+            const R jw = R(4) * wgt, kw = -R(a.hybrid) * wgt;
 #define OUT(ia, ib, ic, id) outv[(((ia)*NB + (ib)) * NC + (ic)) * ND + (id)]
             constexpr int E1 = NAB, E2 = E1 + NCD, E3 = E2 + NA * NC, E4 = E3 + NA * ND, E5 = E4 + NB * NC,
                           E6 = E5 + NB * ND;
             const int eend = a.hybrid != 0.0 ? E6 : E2;
             for (int e = lane; e < eend; e += 64) {
-                double s = 0.0;
+                R s = R(0);
                 long long idx;
-                double wt;
+                R wt;
                 if (e < E1) {
                     const int ia = e / NB, ib = e % NB;
                     for (int ic = 0; ic < NC; ++ic)
@@ -244,7 +283,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     idx = (long long)(fb + ib) * n + fd + id;
                     wt = kw;
                 }
-                fx_add(Xc, idx, wt * s);
+                fx_add(Xc, idx, double(wt * s));
             }
 #undef OUT
             __syncthreads();   // LDS is reused by the next quartet

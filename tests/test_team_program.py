@@ -79,13 +79,17 @@ def run_program(cls, qa, qb, qc, qd, Mb):
     for h in range(na * nb * nf):
         ab, f = divmod(h, nf)
         for u in range(4):
-            e, p3, cf = bra[ab][u]
+            e, p3, cf = bra[ab][u]   # the generator packs each term as e | p3 << 12 | cf << 18
+            packed = e | (p3 << 12) | (cf << 18)
+            assert (packed & 0xfff, (packed >> 12) & 63, (packed >> 18) & 15) == (e, p3, cf)
             hb[h] += cf * pw[p3 & 3] * pw[3 + ((p3 >> 2) & 3)] * pw[6 + ((p3 >> 4) & 3)] * acc[e * nf + f]
     out = np.zeros(na * nb * nc * nd)
     for o in range(out.size):
         ab, cdk = divmod(o, nc * nd)
         for u in range(4):
             f, p3, cf = ket[cdk][u]
+            packed = f | (p3 << 12) | (cf << 18)
+            assert (packed & 0xfff, (packed >> 12) & 63, (packed >> 18) & 15) == (f, p3, cf)
             out[o] += cf * pw[9 + (p3 & 3)] * pw[12 + ((p3 >> 2) & 3)] * pw[15 + ((p3 >> 4) & 3)] * hb[ab * nf + f]
     return out.reshape(na, nb, nc, nd)
 
