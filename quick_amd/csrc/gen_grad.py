@@ -271,12 +271,15 @@ def grad_sets(cls):
             (r(la, la + lb + 1), r(lc, lc + ld)), (r(la, la + lb), r(lc + 1, lc + ld + 1))]
 
 
-def gen_vrr_grad(cls):
+def gen_vrr_grad(cls, mask=15):
+    """vrg_<tag>_<mask>: the raised VRR accumulating the sets in mask (bit 0: S0,
+    1: SA, 2: SB, 3: SC)."""
     from gen_eri import name_of
     sets = grad_sets(cls)
     targets = []
-    for E, F in sets:
-        targets += [(e, f, 0) for e in E for f in F]
+    for s_, (E, F) in enumerate(sets):
+        if mask >> s_ & 1:
+            targets += [(e, f, 0) for e in E for f in F]
     order_, seen = [], set()
 
     def visit(node):
@@ -308,46 +311,54 @@ def gen_vrr_grad(cls):
     for t in targets:
         visit(t)
     X = "xyz"
+    # where each target goes: node -> [(set, index)]
+    where = {}
+    for s_, (E, F) in enumerate(sets):
+        if not mask >> s_ & 1:
+            continue
+        k = 0
+        for e in E:
+            for f in F:
+                where.setdefault((e, f, 0), []).append((s_, k))
+                k += 1
     lines = []
     for node in order_:
         e, f, m = node
         nm = name_of(e, f, m)
         if sum(e) == 0 and sum(f) == 0:
             lines.append(f"    const R {nm} = F[{m}];")
-            continue
-        if sum(f) > 0:
-            i = first_dir(f)
-            f1 = sub(f, i)
-            expr = f"QC{X[i]} * {name_of(e, f1, m)} + WQ{X[i]} * {name_of(e, f1, m + 1)}"
-            if f1[i] > 0:
-                f2 = sub(f1, i)
-                expr += f" + R({f1[i]}) * oo2e * ({name_of(e, f2, m)} - roe * {name_of(e, f2, m + 1)})"
-            if e[i] > 0:
-                expr += f" + R({e[i]}) * oo2ze * {name_of(sub(e, i), f1, m + 1)}"
         else:
-            i = first_dir(e)
-            e1 = sub(e, i)
-            expr = f"PA{X[i]} * {name_of(e1, f, m)} + WP{X[i]} * {name_of(e1, f, m + 1)}"
-            if e1[i] > 0:
-                e2 = sub(e1, i)
-                expr += f" + R({e1[i]}) * oo2z * ({name_of(e2, f, m)} - roz * {name_of(e2, f, m + 1)})"
-        lines.append(f"    const R {nm} = {expr};")
+            if sum(f) > 0:
+                i = first_dir(f)
+                f1 = sub(f, i)
+                expr = f"QC{X[i]} * {name_of(e, f1, m)} + WQ{X[i]} * {name_of(e, f1, m + 1)}"
+                if f1[i] > 0:
+                    f2 = sub(f1, i)
+                    expr += f" + R({f1[i]}) * oo2e * ({name_of(e, f2, m)} - roe * {name_of(e, f2, m + 1)})"
+                if e[i] > 0:
+                    expr += f" + R({e[i]}) * oo2ze * {name_of(sub(e, i), f1, m + 1)}"
+            else:
+                i = first_dir(e)
+                e1 = sub(e, i)
+                expr = f"PA{X[i]} * {name_of(e1, f, m)} + WP{X[i]} * {name_of(e1, f, m + 1)}"
+                if e1[i] > 0:
+                    e2 = sub(e1, i)
+                    expr += f" + R({e1[i]}) * oo2z * ({name_of(e2, f, m)} - roz * {name_of(e2, f, m + 1)})"
+            lines.append(f"    const R {nm} = {expr};")
+        # accumulate a target as soon as it exists, so it need not stay live
+        for s_, k in where.get(node, []):
+            w = ["", "wA * ", "wB * ", "wC * "][s_]
+            lines.append(f"    acc{'0ABC'[s_]}[{k}] += {w}{nm};")
     acc = []
-    for s_, (E, F) in enumerate(sets):
-        w = ["", "wA * ", "wB * ", "wC * "][s_]
-        k = 0
-        for e in E:
-            for f in F:
-                acc.append(f"    acc{'0ABC'[s_]}[{k}] += {w}{name_of(e, f, 0)};")
-                k += 1
-    sig = (f"template <typename R>\n__device__ __forceinline__ void vrg_{tag(cls)}(const R* __restrict__ F, "
+    sig = (f"template <typename R>\n__device__ __forceinline__ void vrg_{tag(cls)}_{mask}(const R* __restrict__ F, "
            "R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, "
            "R oo2z, R oo2e, R oo2ze, R roz, R roe, R wA, R wB, R wC, "
            "R* __restrict__ acc0, R* __restrict__ accA, R* __restrict__ accB, R* __restrict__ accC)")
     return f"{sig}\n{{\n" + "\n".join(lines + acc) + "\n}\n", [len(E) * len(F) for E, F in sets]
 
 
-def gen_grad_contract(cls):
+def gen_grad_contract(cls, mask=15):
+    """gcon_<tag>_<mask>: the derivative terms whose source set is in mask."""
     la, lb, lc, ld = cls
     sets = grad_sets(cls)
     X = "xyz"
@@ -413,6 +424,8 @@ def gen_grad_contract(cls):
     for which, src, sign in stages:
         if sign < 0 and base[which] == 0:
             continue
+        if not mask >> src & 1:
+            continue
         shp = list(base)
         shp[which] += sign
         val = stage_values(src, *shp)
@@ -433,28 +446,69 @@ def gen_grad_contract(cls):
                             vv[which] = tuple(w)
                             term = val(*vv)
                             fac = "" if sign > 0 else (f"R({nx}) * " if nx > 1 else "")
-                            sums[x].append(f"{fac}K[{o}] * {term}")
+                            op = "+=" if sign > 0 else "-="
+                            # summed as soon as the value exists, so it need not stay live
+                            lines.append(f"    g[{3 * which + x}] {op} {fac}K[{o}] * {term};")
                         o += 1
-        for x in range(3):
-            if not sums[x]:
-                continue
-            op = "+=" if sign > 0 else "-="
-            lines.append(f"    g[{3 * which + x}] {op} " + " + ".join(sums[x]) + ";")
-    sig = (f"template <typename R>\n__device__ __forceinline__ void gcon_{tag(cls)}(const R* __restrict__ acc0, "
+    sig = (f"template <typename R>\n__device__ __forceinline__ void gcon_{tag(cls)}_{mask}(const R* __restrict__ acc0, "
            "const R* __restrict__ accA, const R* __restrict__ accB, const R* __restrict__ accC, "
            "R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, const R* __restrict__ K, R* __restrict__ g)")
     return f"{sig}\n{{\n" + "\n".join(lines) + "\n}\n"
 
 
+PASS_BUDGET = 104   # accumulators + density factors (FP64 values) per lane and pass
+
+
+def grad_passes(cls):
+    """Partition of the four sets into passes over the primitive quartets (the
+    VRR is recomputed per pass): the fewest passes whose accumulators plus the
+    NOUT density factors stay within PASS_BUDGET; four single-set passes at most."""
+    sizes = [len(E) * len(F) for E, F in grad_sets(cls)]
+    nout = 1
+    for l in cls:
+        nout *= (l + 1) * (l + 2) // 2
+    for parts in ([[0, 1, 2, 3]], [[0, 1], [2, 3]], [[1], [2], [3], [0]]):
+        if max(sum(sizes[i] for i in p) for p in parts) + nout <= PASS_BUDGET:
+            return parts
+    return [[1], [2], [3], [0]]
+
+
 def write_light(path):
     out = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.",
-           "// Straight-line gradient code of the light classes (vrg_*: raised VRR into four",
-           "// contracted sets; gcon_*: derivative HRR and density contraction).", "#pragma once", ""]
+           "// Straight-line gradient code of the light classes: vrg_<class>_<mask> (raised VRR",
+           "// into the contracted sets in mask), gcon_<class>_<mask> (derivative HRR and density",
+           "// contraction of the terms those sets feed), and the passes a kernel makes.", "#pragma once", ""]
     for cls in LIGHT_CLASSES:
-        v, sizes = gen_vrr_grad(cls)
-        out += [v, gen_grad_contract(cls)]
-        out.append(f"struct GradSets_{tag(cls)} {{ static constexpr int n0 = {sizes[0]}, nA = {sizes[1]}, "
-                   f"nB = {sizes[2]}, nC = {sizes[3]}; }};")
+        t = tag(cls)
+        passes = grad_passes(cls)
+        masks = [sum(1 << i for i in p) for p in passes]
+        sizes = [len(E) * len(F) for E, F in grad_sets(cls)]
+        for m in masks:
+            v, _ = gen_vrr_grad(cls, m)
+            out += [v, gen_grad_contract(cls, m)]
+        width = max(sum(sizes[i] for i in p) for p in passes)
+        out.append(f"struct GradSets_{t} {{")
+        out.append(f"    static constexpr int n0 = {sizes[0]}, nA = {sizes[1]}, nB = {sizes[2]}, nC = {sizes[3]};")
+        out.append(f"    static constexpr int kPasses = {len(masks)}, kWidth = {width};")
+        out.append(f"    static constexpr int kMask[4] = {{{', '.join(str(m) for m in masks + [0] * (4 - len(masks)))}}};")
+        out.append("};")
+        # dispatch on the mask at compile time
+        out.append("template <int M, typename R>")
+        out.append(f"__device__ __forceinline__ void vrg_{t}(const R* F, R PAx, R PAy, R PAz, R WPx, R WPy, R WPz, "
+                   "R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, R oo2z, R oo2e, R oo2ze, R roz, R roe, R wA, R wB, "
+                   "R wC, R* a0, R* aA, R* aB, R* aC) {")
+        for k, m in enumerate(masks):
+            kw = "if" if k == 0 else "else if"
+            out.append(f"    {kw} constexpr (M == {m}) vrg_{t}_{m}<R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, "
+                       "QCz, WQx, WQy, WQz, oo2z, oo2e, oo2ze, roz, roe, wA, wB, wC, a0, aA, aB, aC);")
+        out.append("}")
+        out.append("template <int M, typename R>")
+        out.append(f"__device__ __forceinline__ void gcon_{t}(const R* a0, const R* aA, const R* aB, const R* aC, "
+                   "R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, const R* K, R* g) {")
+        for k, m in enumerate(masks):
+            kw = "if" if k == 0 else "else if"
+            out.append(f"    {kw} constexpr (M == {m}) gcon_{t}_{m}<R>(a0, aA, aB, aC, ABx, ABy, ABz, CDx, CDy, CDz, K, g);")
+        out.append("}")
     new = "\n".join(out) + "\n"
     if not os.path.exists(path) or open(path).read() != new:
         open(path, "w").write(new)

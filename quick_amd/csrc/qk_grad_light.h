@@ -20,19 +20,19 @@ struct GradLight;
     template <>                                                                                              \
     struct GradLight<la, lb, lc, ld> {                                                                       \
         using Sets = GradSets_##TAG;                                                                         \
-        template <typename R>                                                                                \
+        template <int M, typename R>                                                                         \
         static __device__ __forceinline__ void vrg(const R* F, R PAx, R PAy, R PAz, R WPx, R WPy, R WPz,   \
                                                    R QCx, R QCy, R QCz, R WQx, R WQy, R WQz, R oo2z, R oo2e, \
                                                    R oo2ze, R roz, R roe, R wA, R wB, R wC, R* a0, R* aA,   \
                                                    R* aB, R* aC) {                                           \
-            vrg_##TAG<R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, oo2z, oo2e, oo2ze,  \
-                         roz, roe, wA, wB, wC, a0, aA, aB, aC);                                              \
+            vrg_##TAG<M, R>(F, PAx, PAy, PAz, WPx, WPy, WPz, QCx, QCy, QCz, WQx, WQy, WQz, oo2z, oo2e, oo2ze, \
+                            roz, roe, wA, wB, wC, a0, aA, aB, aC);                                           \
         }                                                                                                    \
-        template <typename R>                                                                                \
+        template <int M, typename R>                                                                         \
         static __device__ __forceinline__ void gcon(const R* a0, const R* aA, const R* aB, const R* aC,     \
                                                     R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, const R* K,   \
                                                     R* g) {                                                  \
-            gcon_##TAG<R>(a0, aA, aB, aC, ABx, ABy, ABz, CDx, CDy, CDz, K, g);                              \
+            gcon_##TAG<M, R>(a0, aA, aB, aC, ABx, ABy, ABz, CDx, CDy, CDz, K, g);                           \
         }                                                                                                    \
     };
 QK_GRAD_LIGHT(0, 0, 0, 0, ssss)
@@ -50,11 +50,13 @@ QK_GRAD_LIGHT(2, 1, 1, 0, dpps)
 QK_GRAD_LIGHT(2, 2, 0, 0, ddss)
 #undef QK_GRAD_LIGHT
 
-// one wavefront per SIMD for the classes whose four sets exceed ~100 values
+// Two wavefronts per SIMD when one pass's accumulators and density factors fit
+// the budget of the generator (no spills), else one.
 template <int LA, int LB, int LC, int LD>
 struct GradLightOcc {
     using S = typename GradLight<LA, LB, LC, LD>::Sets;
-    static constexpr int value = S::n0 + S::nA + S::nB + S::nC <= 100 ? 2 : 1;
+    static constexpr int nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+    static constexpr int value = S::kWidth + nout <= 104 ? 2 : 1;
 };
 
 // primitive pairs of the bra for the gradient: the Fock slab's seven fields
@@ -129,56 +131,12 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
             active = ct > a.cutoff && ct * dn > a.cutoff;
         }
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra_grad(a, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
+        stage_bra_grad(a, pb0, pbs, nbp, lane, bra);
         if (!active) continue;
         ++nsurv;
         const int Mb = a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd] + 2;
         const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
         const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
-        double acc0[n0], accA[nA], accB[nB], accC[nC];
-#pragma unroll
-        for (int k = 0; k < n0; ++k) acc0[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < nA; ++k) accA[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < nB; ++k) accB[k] = 0.0;
-#pragma unroll
-        for (int k = 0; k < nC; ++k) accC[k] = 0.0;
-        const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
-        for (int i0 = 0; i0 < nbp; i0 += 64) {
-            const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
-            if (i0 > 0) stage_bra_grad(a, pb0 + i0 * pbs, pbs, ni, lane, bra);
-            for (int j = 0; j < nkp; ++j) {
-                const int kj = pk0 + j * pks;
-                const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
-                const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
-                const double Kcd = a.pp.K[kj], ccd = a.pp.cut[kj], wC = 2.0 * a.ea[kj];
-                const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
-                for (int i = 0; i < ni; ++i) {
-                    if (!((dn * bra[6 * 64 + i]) * ccd > a.cutoff)) continue;
-                    ++nprimq;
-                    const double zab = bra[i], oo2z = bra[64 + i];
-                    const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                    const double inv = 1.0 / (zab + zcd);
-                    const double roz = zcd * inv, roe = zab * inv;
-                    const double ROU = zab * roz;
-                    const double Wx = (Px * zab + Qx * zcd) * inv;
-                    const double Wy = (Py * zab + Qy * zcd) * inv;
-                    const double Wz = (Pz * zab + Qz * zcd) * inv;
-                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                    double F[LT2 + 1];
-                    boys_fit<LT2>(T, Mb, s_boys, F);
-                    const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
-#pragma unroll
-                    for (int m = 0; m <= LT2; ++m) F[m] *= s;
-                    Ops::template vrg<double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx, QCy, QCz,
-                                              Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5 * inv, roz, roe,
-                                              2.0 * bra[7 * 64 + i], 2.0 * bra[8 * 64 + i], wC, acc0, accA, accB,
-                                              accC);
-                }
-            }
-        }
         // density factors K = w (4 D_ab D_cd - h (D_ac D_bd + D_ad D_bc)), cons folded into D
         const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
         double wq = 1.0;
@@ -204,11 +162,67 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
         double g[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[k] = 0.0;
-        {
-            const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
-            const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
-            Ops::template gcon<double>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy, Cz - Dz,
-                                       K, g);
+        const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
+        const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+        const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
+        // one pass over the primitive quartets per group of sets (the VRR is
+        // recomputed; the sets of one pass are the only accumulators live)
+        auto pass = [&](auto mtag) {
+            constexpr int M = decltype(mtag)::value;
+            double acc0[(M & 1) ? n0 : 1], accA[(M & 2) ? nA : 1], accB[(M & 4) ? nB : 1], accC[(M & 8) ? nC : 1];
+#pragma unroll
+            for (int k = 0; k < ((M & 1) ? n0 : 1); ++k) acc0[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < ((M & 2) ? nA : 1); ++k) accA[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < ((M & 4) ? nB : 1); ++k) accB[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < ((M & 8) ? nC : 1); ++k) accC[k] = 0.0;
+            {
+                const int ni = nbp;   // <= 64 primitive pairs per Qshell pair (checked on the host)
+                for (int j = 0; j < nkp; ++j) {
+                    const int kj = pk0 + j * pks;
+                    const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
+                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
+                    const double Kcd = a.pp.K[kj], ccd = a.pp.cut[kj], wC = 2.0 * a.ea[kj];
+                    const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
+                    for (int i = 0; i < ni; ++i) {
+                        if (!((dn * bra[6 * 64 + i]) * ccd > a.cutoff)) continue;
+                        if (M == Ops::Sets::kMask[0]) ++nprimq;
+                        const double zab = bra[i], oo2z = bra[64 + i];
+                        const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
+                        const double inv = 1.0 / (zab + zcd);
+                        const double roz = zcd * inv, roe = zab * inv;
+                        const double ROU = zab * roz;
+                        const double Wx = (Px * zab + Qx * zcd) * inv;
+                        const double Wy = (Py * zab + Qy * zcd) * inv;
+                        const double Wz = (Pz * zab + Qz * zcd) * inv;
+                        const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                        const double T = (dx * dx + dy * dy + dz * dz) * ROU;
+                        double F[LT2 + 1];
+                        boys_fit<LT2>(T, Mb, s_boys, F);
+                        const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
+#pragma unroll
+                        for (int m = 0; m <= LT2; ++m) F[m] *= s;
+                        Ops::template vrg<M, double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx,
+                                                     QCy, QCz, Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5 * inv,
+                                                     roz, roe, 2.0 * bra[7 * 64 + i], 2.0 * bra[8 * 64 + i], wC,
+                                                     acc0, accA, accB, accC);
+                    }
+                }
+            }
+            Ops::template gcon<M, double>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
+                                          Cz - Dz, K, g);
+        };
+        pass(std::integral_constant<int, Ops::Sets::kMask[0]>{});
+        if constexpr (Ops::Sets::kPasses > 1) {
+            pass(std::integral_constant<int, Ops::Sets::kMask[1]>{});
+        }
+        if constexpr (Ops::Sets::kPasses > 2) {
+            pass(std::integral_constant<int, Ops::Sets::kMask[2]>{});
+        }
+        if constexpr (Ops::Sets::kPasses > 3) {
+            pass(std::integral_constant<int, Ops::Sets::kMask[3]>{});
         }
 #pragma unroll
         for (int x = 0; x < 3; ++x) {

@@ -394,6 +394,11 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
             set_err("shell %d: primitive range %d+%d outside 1..%d", s + 1, kstart[s], kprim[s], jbasis);
             return -3;
         }
+        if (kprim[s] > 8) {   // a wavefront holds one bra's primitive pairs, one per lane
+            set_err("shell %d: %d primitives; this build supports at most 8 per shell (64 primitive pairs)",
+                    s + 1, kprim[s]);
+            return -3;
+        }
         if (katom[s] < 1 || katom[s] > natom) {
             set_err("shell %d: atom %d outside 1..%d", s + 1, katom[s], natom);
             return -3;

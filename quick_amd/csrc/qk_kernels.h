@@ -211,12 +211,12 @@ QK_CLASS(2, 2, 2, 0, ddds)
 QK_CLASS(2, 2, 2, 1, dddp)
 QK_CLASS(2, 2, 2, 2, dddd)
 
-// Blocks of 256 threads kept resident per CU: light classes are pushed to a
-// higher occupancy (fewer VGPRs), the heavy d classes keep the full register file.
+// Wavefronts per SIMD the light kernels are compiled for: two (256 VGPRs, no
+// spills for the small accumulators), one for the larger classes.
 template <int LA, int LB, int LC, int LD>
 struct Occupancy {
     static constexpr int nacc = nrange(LA, LA + LB) * nrange(LC, LC + LD);
-    static constexpr int value = nacc <= 9 ? 4 : (nacc <= 36 ? 2 : 1);
+    static constexpr int value = nacc <= 36 ? 2 : 1;
 };
 
 // int64 sum over the 64 lanes of a wavefront (exact, so order-free)
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
     for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
     __syncthreads();
-    // the current bra's primitive pairs (up to 64 at a time), one slab per wavefront
+    // the current bra's primitive pairs (at most 64), one slab per wavefront
     __shared__ double s_bra[4][7 * 64];
     double* bra = s_bra[threadIdx.x >> 6];
 
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
 
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra(a.pp, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
+        stage_bra(a.pp, pb0, pbs, nbp, lane, bra);
 
         if (active) {
             ++nsurv;
@@ -307,9 +307,8 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
             // current one runs), bra primitives inner, read from the lanes that hold
             // them (v_readlane: no memory access in the inner loop)
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
-            for (int i0 = 0; i0 < nbp; i0 += 64) {
-                const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
-                if (i0 > 0) stage_bra(a.pp, pb0 + i0 * pbs, pbs, ni, lane, bra);
+            {
+                const int ni = nbp;   // <= 64 primitive pairs per Qshell pair (checked on the host)
                 double nz = a.pp.zeta[pk0], no = a.pp.oo2z[pk0], nx = a.pp.Px[pk0], ny = a.pp.Py[pk0],
                        nzz = a.pp.Pz[pk0], nK = a.pp.K[pk0], nc = a.pp.cut[pk0];
                 for (int j = 0; j < nkp; ++j) {
@@ -541,7 +540,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0f;
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra(a.ppf, pb0, pbs, nbp < 64 ? nbp : 64, lane, bra);
+        stage_bra(a.ppf, pb0, pbs, nbp, lane, bra);
 
         if (active) {
             ++nsurv;
@@ -553,9 +552,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
             for (int k = 0; k < NACC; ++k) acc[k] = 0.0f;
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
             const float dnf = float(dn);
-            for (int i0 = 0; i0 < nbp; i0 += 64) {
-                const int ni = nbp - i0 < 64 ? nbp - i0 : 64;
-                if (i0 > 0) stage_bra(a.ppf, pb0 + i0 * pbs, pbs, ni, lane, bra);
+            {
+                const int ni = nbp;   // <= 64 primitive pairs per Qshell pair (checked on the host)
                 float nz = a.ppf.zeta[pk0], no = a.ppf.oo2z[pk0], nx = a.ppf.Px[pk0], ny = a.ppf.Py[pk0],
                       nzz = a.ppf.Pz[pk0], nK = a.ppf.K[pk0], nc = a.ppf.cut[pk0];
                 for (int j = 0; j < nkp; ++j) {
