@@ -30,20 +30,32 @@ def shim(tmp_path_factory):
     inc = CSRC / "grad_light.inc"
     gen_grad.write_light(str(inc))
     d = tmp_path_factory.mktemp("gl")
-    src = ["#define __device__", "#define __forceinline__ inline", '#include "grad_light.inc"']
+    src = ["#define __device__", "#define __forceinline__ inline", '#include "grad_light.inc"', """
+// One pass over the primitive quartets for the contracted sets named by mask M
+// (bit 0: S0, 1: SA, 2: SB, 3: SC), then its contraction into g; the device
+// kernel runs the passes of GradSets_<class>::kMask in the same way.
+template <class S, int M, class VRG, class GCON>
+static void one_pass(int np, const double* F, int nf, const double* geo, const double* wts, const double* ABCD,
+                     const double* K, double* g, VRG vrg, GCON gcon) {
+    double a0[S::n0] = {}, aA[S::nA] = {}, aB[S::nB] = {}, aC[S::nC] = {};
+    for (int p = 0; p < np; ++p) {
+        const double* G = geo + 17 * p;
+        vrg(F + nf * p, G[0], G[1], G[2], G[3], G[4], G[5], G[6], G[7], G[8], G[9], G[10], G[11],
+            G[12], G[13], G[14], G[15], G[16], wts[3 * p], wts[3 * p + 1], wts[3 * p + 2], a0, aA, aB, aC);
+    }
+    gcon(a0, aA, aB, aC, ABCD[0], ABCD[1], ABCD[2], ABCD[3], ABCD[4], ABCD[5], K, g);
+}"""]
     for cls in gen_grad.LIGHT_CLASSES:
         t = gen_eri.tag(cls)
+        passes = "\n".join(
+            f"    if constexpr (S::kPasses > {k}) one_pass<S, S::kMask[{k}]>(np, F, nf, geo, wts, ABCD, K, g, "
+            f"vrg_{t}<S::kMask[{k}], double>, gcon_{t}<S::kMask[{k}], double>);" for k in range(4))
         src.append(f"""extern "C" void run_{t}(int np, const double* F, const double* geo, const double* wts,
                         const double* ABCD, const double* K, double* g) {{
-    double a0[GradSets_{t}::n0] = {{}}, aA[GradSets_{t}::nA] = {{}}, aB[GradSets_{t}::nB] = {{}}, aC[GradSets_{t}::nC] = {{}};
+    using S = GradSets_{t};
     const int nf = {sum(cls) + 3};
-    for (int p = 0; p < np; ++p) {{
-        const double* G = geo + 17 * p;
-        vrg_{t}<double>(F + nf * p, G[0], G[1], G[2], G[3], G[4], G[5], G[6], G[7], G[8], G[9], G[10], G[11],
-                        G[12], G[13], G[14], G[15], G[16], wts[3 * p], wts[3 * p + 1], wts[3 * p + 2], a0, aA, aB, aC);
-    }}
     for (int k = 0; k < 9; ++k) g[k] = 0.0;
-    gcon_{t}<double>(a0, aA, aB, aC, ABCD[0], ABCD[1], ABCD[2], ABCD[3], ABCD[4], ABCD[5], K, g);
+{passes}
 }}""")
     (d / "shim.cpp").write_text("\n".join(src) + "\n")
     so = d / "shim.so"
