@@ -22,7 +22,7 @@ Legs:
     oracle/_ref) timed on a bounded sample of the same build (cpu_baseline).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                       [--thresholds 1e-2,1e-3,...|none] [--copy-kernel-mib N]
+                       [--thresholds 1e-3,1e-4,...|none] [--copy-kernel-mib N]
 """
 from __future__ import annotations
 
@@ -87,14 +87,20 @@ def cpu_baseline(symbols, xyz_ang, D, stride=16, timeout=300):
             "seconds_per_build_est": est}
 
 
+def kernel_of(tag):
+    """name of the Fock kernel that evaluates quartet class `tag` (as rocprofv3 lists it, short form)"""
+    return "team_kernel" if tag in ("dpdp", "ddpp", "ddds", "dddp", "dddd") else "fock_kernel"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--thresholds", default="1e-2,1e-3,1e-4,1e-5,1e-6",
+    ap.add_argument("--thresholds", default="1e-3,1e-4,1e-5,1e-6",
                     help="FP32 thresholds of the mixed-precision sweep ('none': FP64 only)")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the gradient legs")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), to "
                          "calibrate the FETCH_SIZE/WRITE_SIZE counters, and exit")
@@ -151,8 +157,8 @@ def main():
         ms = ev0.elapsed_time(ev1) / args.steps
         log(f"{label}: {args.steps} builds {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build")
         # kernel rows: (kernel, tag, quartets, prim quartets, flops, ms, precision)
-        rows = [(f"fock_kernel<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp64") for c in eng.class_stats()]
-        rows += [(f"fock_kernel_f32<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp32")
+        rows = [(f"{kernel_of(c[0])}<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp64") for c in eng.class_stats()]
+        rows += [(f"{kernel_of(c[0])}_f32<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp32")
                  for c in eng.class_stats_fp32()]
         rows = [r for r in rows if r[2]]
         G = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
@@ -190,6 +196,7 @@ def main():
     for r in sweep:
         r.pop("_rows"), r.pop("_st")
 
+    thr = pick["fp32_threshold"] if pick is not None else None
     dom = max(rows, key=lambda r: r[5])
     kname, tag, nq, npq, flops, kms, kprec = dom
     peak = PEAK_FP64_TFLOPS if kprec == "fp64" else PEAK_FP32_TFLOPS
@@ -202,7 +209,7 @@ def main():
     # nuclear gradient (four-centre term, FP64): the same (H2O)32 system and
     # n-icosane C20H42 (510 bf), dE2/dR at the fixed synthetic density
     grads = {}
-    for gname in ("water32", "icosane"):
+    for gname in (() if args.no_gradient else ("water32", "icosane")):
         if gname == "water32":
             gb, gD, ge = b, D, eng
         else:
@@ -234,17 +241,21 @@ def main():
                         "precision": "fp64"}
         log(f"gradient {gname}: {gs * 1e3:.2f} ms ({grads[gname]['quartets']} quartets)")
 
+    # HBM bytes per launch of the roofline kernel: FETCH_SIZE + WRITE_SIZE from
+    # rocprofv3 --pmc passes over this workload, scaled by a plain copy's
+    # readings (profiles/pmc_traffic_water32.json); null when the file has no
+    # figure for this kernel at this precision and threshold
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic_water32.json"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get(kname)
-        except Exception:
-            traffic = None
+        tj = json.loads(tfile.read_text())
+        if precision == "mixed" and thr == tj.get("fp32_threshold"):
+            traffic = tj["mixed_build"].get(kname, {}).get("bytes")
+        elif precision == "fp64":
+            traffic = tj["fp64_build"].get(kname, {}).get("bytes")
 
     base = None if args.no_cpu_baseline else cpu_baseline(syms, xyz, D)
     value = 1000.0 / ms
-    thr = pick["fp32_threshold"] if pick is not None else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -265,7 +276,8 @@ def main():
                    "precision": precision, "fp32_threshold": thr, "geometry_seed": 2,
                    "density_seed": DENSITY_SEED, "quartets": int(st.quartets),
                    "quartets_fp32": int(st.quartets_fp32), "prim_quartets": int(st.prim_quartets),
-                   "E2_half_trDG_fp64": E64, "gradient_s": grads["water32"]["seconds"]},
+                   "E2_half_trDG_fp64": E64,
+                   "gradient_s": grads["water32"]["seconds"] if grads else None},
         "gradient": grads,
         "fp64": {"builds_per_s": 1000.0 / ms64, "ms_per_build": ms64},
         "mixed_vs_fp64_speedup": ms64 / ms,
@@ -273,7 +285,7 @@ def main():
         "eri_quartets_per_s": st.quartets * value,
         "achieved_tflops_total": (total64 + total32) / (ms * 1e-3) / 1e12,
         "vector_roofline_time_frac": t_peak_ms / ms,
-        "roofline": {"bound": "valu-" + kprec, "kernel": kname, "achieved": achieved,
+        "roofline": {"bound": "mfma", "pipe": "VALU " + kprec, "kernel": kname, "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": kms},
         "cpu_baseline": base,

@@ -114,3 +114,13 @@ def test_team_program_matches_oracle(golden, cls):
             i, j, k, l = (f0[t] + idx[t] for t in range(4))
             ref = o.eri(i, j, k, l) / (fx.cons[i] * fx.cons[j] * fx.cons[k] * fx.cons[l])
             assert abs(got[idx] - ref) <= 1e-11 * max(1.0, abs(ref)), (cls, sh, idx, got[idx], ref)
+
+
+def test_bench_names_the_team_kernels():
+    """bench.py labels each class's kernel by name (for the rocprofv3 figures);
+    the classes it calls team_kernel are exactly the generator's team classes."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    for cls in gen_eri.quartet_classes():
+        want = "team_kernel" if cls in gen_eri.TEAM_CLASSES else "fock_kernel"
+        assert bench.kernel_of(gen_eri.tag(cls)) == want, cls
