@@ -195,8 +195,8 @@ def main():
         ms, rows, st, precision = ms64, rows64, st64, "fp64"
     for r in sweep:
         r.pop("_rows"), r.pop("_st")
-
     thr = pick["fp32_threshold"] if pick is not None else None
+
     dom = max(rows, key=lambda r: r[5])
     kname, tag, nq, npq, flops, kms, kprec = dom
     peak = PEAK_FP64_TFLOPS if kprec == "fp64" else PEAK_FP32_TFLOPS
@@ -206,9 +206,12 @@ def main():
     # mixed-mode bound: time at peak for the FP64 and the FP32 flops
     t_peak_ms = (total64 / (PEAK_FP64_TFLOPS * 1e12) + total32 / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
 
-    # nuclear gradient (four-centre term, FP64): the same (H2O)32 system and
-    # n-icosane C20H42 (510 bf), dE2/dR at the fixed synthetic density
+    # nuclear gradient (four-centre term): the same (H2O)32 system and
+    # n-icosane C20H42 (510 bf), dE2/dR at the fixed synthetic density, in FP64
+    # and in mixed precision (FP32 below the picked Fock threshold; 1e-3 when
+    # the FP64 build was picked), the mixed one checked against FP64
     grads = {}
+    gthr = thr if thr is not None else 1e-3
     for gname in (() if args.no_gradient else ("water32", "icosane")):
         if gname == "water32":
             gb, gD, ge = b, D, eng
@@ -219,27 +222,44 @@ def main():
             ge = FockEngine(gb)
             ge.compute_schwarz()
         gDd = torch.from_numpy(np.ascontiguousarray(gD.ravel(order="F"))).to(dev)
-        gout = torch.zeros(3 * ge.natom, dtype=torch.float64, device=dev)
-        gopts = make_opts(precision=QK_FP64)
-        ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)       # warm-up (builds the work lists)
-        torch.cuda.synchronize()
-        ge.set_timing(True)
-        nrep = max(1, min(args.steps, 5))
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(nrep):
-            ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        gs = e0.elapsed_time(e1) / nrep / 1e3
-        gst = ge.grad_class_stats()
-        for c in sorted(gst, key=lambda c: -c[3]):
-            if c[1]:
-                log(f"  grad class {c[0]}: {c[1]:>10d} quartets {c[2]:>11d} prim {c[3]:8.3f} ms")
-        grads[gname] = {"seconds": gs, "atoms": int(ge.natom), "nbasis": int(gb.nbasis),
-                        "quartets": int(sum(c[1] for c in gst)), "prim_quartets": int(sum(c[2] for c in gst)),
-                        "precision": "fp64"}
-        log(f"gradient {gname}: {gs * 1e3:.2f} ms ({grads[gname]['quartets']} quartets)")
+        res = {}
+        for gprec, gopts in (("fp64", make_opts(precision=QK_FP64)),
+                             ("mixed", make_opts(precision=QK_MIXED, fp32_threshold=gthr))):
+            gout = torch.zeros(3 * ge.natom, dtype=torch.float64, device=dev)
+            ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)   # warm-up (builds the work lists)
+            torch.cuda.synchronize()
+            gval = gout.cpu().numpy()
+            ge.set_timing(True)
+            nrep = max(1, min(args.steps, 5))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(nrep):
+                ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            gs = e0.elapsed_time(e1) / nrep / 1e3
+            st64, st32 = ge.grad_class_stats(), ge.grad_class_stats(fp32=True)
+            for lab, gst in (("", st64), (" fp32", st32)):
+                for c in sorted(gst, key=lambda c: -c[3]):
+                    if c[1]:
+                        log(f"  grad {gprec}{lab} class {c[0]}: {c[1]:>10d} quartets {c[2]:>11d} prim {c[3]:8.3f} ms")
+            q64, q32 = sum(c[1] for c in st64), sum(c[1] for c in st32)
+            r = {"seconds": gs, "atoms": int(ge.natom), "nbasis": int(gb.nbasis), "quartets": int(q64 + q32),
+                 "prim_quartets": int(sum(c[2] for c in st64) + sum(c[2] for c in st32)), "precision": gprec}
+            if gprec == "mixed":
+                r.update({"fp32_threshold": gthr, "fp32_quartet_share": q32 / max(q64 + q32, 1),
+                          "max_abs_dgrad_vs_fp64": float(np.abs(gval - res["fp64"]["_g"]).max())})
+            r["_g"] = gval
+            res[gprec] = r
+            log(f"gradient {gname} {gprec}: {gs * 1e3:.2f} ms ({q64 + q32} quartets"
+                + (f", fp32 share {r['fp32_quartet_share']:.3f}, max|dgrad| {r['max_abs_dgrad_vs_fp64']:.2e})"
+                   if gprec == "mixed" else ")"))
+        for r in res.values():
+            r.pop("_g")
+        grads[gname] = {"seconds": res["fp64"]["seconds"], "atoms": res["fp64"]["atoms"],
+                        "nbasis": res["fp64"]["nbasis"], "quartets": res["fp64"]["quartets"],
+                        "prim_quartets": res["fp64"]["prim_quartets"], "precision": "fp64",
+                        "mixed": res["mixed"], "mixed_vs_fp64_speedup": res["fp64"]["seconds"] / res["mixed"]["seconds"]}
 
     # HBM bytes per launch of the roofline kernel: FETCH_SIZE + WRITE_SIZE from
     # rocprofv3 --pmc passes over this workload, scaled by a plain copy's
@@ -279,6 +299,11 @@ def main():
                    "E2_half_trDG_fp64": E64,
                    "gradient_s": grads["water32"]["seconds"] if grads else None},
         "gradient": grads,
+        "fock_plus_gradient_water32": None if "water32" not in grads else {
+            "fp64_ms": ms64 + grads["water32"]["seconds"] * 1e3,
+            "mixed_ms": ms + grads["water32"]["mixed"]["seconds"] * 1e3,
+            "mixed_vs_fp64_speedup": (ms64 + grads["water32"]["seconds"] * 1e3) /
+                                     (ms + grads["water32"]["mixed"]["seconds"] * 1e3)},
         "fp64": {"builds_per_s": 1000.0 / ms64, "ms_per_build": ms64},
         "mixed_vs_fp64_speedup": ms64 / ms,
         "mixed_sweep": sweep,

@@ -136,7 +136,9 @@ int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 /* Backward: dgrad (3*natom, device) <- dgrad + dE2/dR at fixed density dD,
  * E2 = 1/2 sum D G(D): the four-centre term of hfgrad (gradient.f90:261-320,
  * shell.f90:1863-2535, hrr.f90:862-1014).  opts->tau is the gradient cutoff
- * (quick_method%gradCutoff); FP64 only.  Runs on `stream`. */
+ * (quick_method%gradCutoff).  With opts->precision QK_MIXED, quartets whose
+ * Schwarz bound Y_ij*Y_kl lies below opts->fp32_threshold are differentiated
+ * in FP32 (accumulated in FP64 fixed point).  Runs on `stream`. */
 int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
 int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 
@@ -167,6 +169,8 @@ int qk_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quarte
 /* Surviving quartets and primitive quartets of class k in the last gradient,
  * and its kernel's average time in ms (0 unless timing is on). */
 int qk_grad_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, float* ms);
+/* The same for the FP32 part of the last mixed-precision gradient. */
+int qk_grad_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, float* ms);
 /* The same figures for class k's FP32 kernel (QK_MIXED; zero otherwise). */
 int qk_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
 

@@ -522,7 +522,10 @@ def write_light_units(outdir):
         body = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.", '#include "../qk_grad_light.h"',
                 "namespace qk {",
                 f"void launch_grad_light_{t}(const GradArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{",
-                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), lds, s, a, w);",
+                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, double>), dim3(grid), dim3(256), lds, s, a, w);",
+                "}",
+                f"void launch_grad_light_f32_{t}(const GradArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{",
+                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, float>), dim3(grid), dim3(256), lds, s, a, w);",
                 "}", "}  // namespace qk"]
         fn = os.path.join(outdir, f"grad_{t}.hip")
         new = "\n".join(body) + "\n"

@@ -28,6 +28,7 @@ namespace qk {
 // nf <= kMaxF: the same branches and recursions as boys_ref, F_0..F_nf
 // produced (loops unrolled to kMaxF so that F stays in registers).
 constexpr int kMaxF = 10;
+static_assert(kMaxF == kMaxGradF, "FP32 Boys orders of the gradient");
 __device__ __forceinline__ void boys_ref_n(double X, int Mb, int nf, double* F) {
     const double E = exp(-X);
     const bool up = (X > 3.0e-5) || (X > 3.0e-7 && Mb < 4);
@@ -63,9 +64,13 @@ __device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, dou
     if (iv != 0) atomicAdd(G + idx, (unsigned long long)iv);
 }
 
-// kGradTeam threads (one or four wavefronts) work on one quartet
-template <int kGradTeam>
+// kGradTeam threads (one or four wavefronts) work on one quartet.  R = double:
+// QUICK's Boys fit; R = float (mixed precision, weak quartets): the tabulated
+// FP32 Boys expansion, FP32 LDS buffers and arithmetic, the nine sums promoted
+// to FP64 before the wavefront reduction.
+template <int kGradTeam, typename R>
 __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+    constexpr bool f64 = sizeof(R) == 8;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;          // thread of the block (kGradTeam threads share one quartet)
     const int n = a.n;
@@ -78,13 +83,18 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
     const uint4* nodes = reinterpret_cast<const uint4*>(tab + tab[kGhOffNodes]);
     const int* lev = tab + tab[kGhOffLev];
     const int* tgt = tab + tab[kGhOffTgt];
-    double* V = lds;              // VRR slots; after the primitive loop the stage buffers
-    double* S = lds + RV;         // S0 | SA | SB | SC
-    double* Kv = S + nS;          // density factor per function quartet
-    double* pw = Kv + NOUT;       // pw[axis*4+k] = AB_axis^k, pw[12+axis*4+k] = CD_axis^k
-    double* Fl = pw + 24;
-    double* gred = Fl + 16;       // per-wavefront partial sums of the nine derivatives
-    const double X0 = 34.98683665524972;
+    double* gred = lds;           // per-wavefront partial sums of the nine derivatives (9 x 4)
+    R* V = reinterpret_cast<R*>(lds + 36);   // VRR slots; after the primitive loop the stage buffers
+    R* S = V + RV;                // S0 | SA | SB | SC
+    R* Kv = S + nS;               // density factor per function quartet
+    R* pw = Kv + NOUT;            // pw[axis*4+k] = AB_axis^k, pw[12+axis*4+k] = CD_axis^k
+    R* Fl = pw + 24;
+    const auto& PP = grad_prims<R>(a);
+    const R* ea = grad_ea<R>(a);
+    const R* eb = grad_eb<R>(a);
+    const R* Dp = grad_density<R>(a);
+    const R X0 = R(34.98683665524972);
+    const R rcut = R(a.cutoff), hyb = R(a.hybrid);
     unsigned long long nsurv = 0, nprimq = 0;
 
     for (long long t = blockIdx.x; t < w.ntiles; t += gridDim.x) {
@@ -103,47 +113,48 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
             if (!(ct > a.cutoff && ct * dn > a.cutoff)) continue;   // uniform across the wavefront
             ++nsurv;
             const int Mb = a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd] + 2;
-            const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
-            const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+            const R Ax = R(a.xyz[3 * atA]), Ay = R(a.xyz[3 * atA + 1]), Az = R(a.xyz[3 * atA + 2]);
+            const R Cx = R(a.xyz[3 * atC]), Cy = R(a.xyz[3 * atC + 1]), Cz = R(a.xyz[3 * atC + 2]);
             __syncthreads();                       // previous quartet's readers of S/V are done
-            for (int k = lane; k < nS; k += kGradTeam) S[k] = 0.0;
-            if (lane == 0) V[0] = 0.0;
+            for (int k = lane; k < nS; k += kGradTeam) S[k] = R(0);
+            if (lane == 0) V[0] = R(0);
 
             const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
             for (int i = 0; i < nbp; ++i) {
                 const int bi = pb0 + i * pbs;
-                const double zab = a.pp.zeta[bi], oo2z = a.pp.oo2z[bi];
-                const double Px = a.pp.Px[bi], Py = a.pp.Py[bi], Pz = a.pp.Pz[bi];
-                const double X2 = X0 * a.pp.K[bi];
-                const double cp1 = dn * a.pp.cut[bi];
-                const double wA = 2.0 * a.ea[bi], wB = 2.0 * a.eb[bi];
+                const R zab = PP.zeta[bi], oo2z = PP.oo2z[bi];
+                const R Px = PP.Px[bi], Py = PP.Py[bi], Pz = PP.Pz[bi];
+                const R X2 = X0 * PP.K[bi];
+                const R cp1 = R(dn) * PP.cut[bi];
+                const R wA = R(2) * ea[bi], wB = R(2) * eb[bi];
                 for (int j = 0; j < nkp; ++j) {
                     const int kj = pk0 + j * pks;
-                    if (!(cp1 * a.pp.cut[kj] > a.cutoff)) continue;
+                    if (!(cp1 * PP.cut[kj] > rcut)) continue;
                     ++nprimq;
-                    const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
-                    const double wC = 2.0 * a.ea[kj];
-                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
-                    const double inv = 1.0 / (zab + zcd);
-                    const double roz = zcd * inv, roe = zab * inv;
-                    const double ROU = zab * roz;
-                    const double Wx = (Px * zab + Qx * zcd) * inv;
-                    const double Wy = (Py * zab + Qy * zcd) * inv;
-                    const double Wz = (Pz * zab + Qz * zcd) * inv;
-                    const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                    const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                    double F[kMaxF + 1];
-                    boys_ref_n(T, Mb, LT2, F);
-                    const double s = X2 * a.pp.K[kj] * sqrt(inv);
-                    double fsel = F[0] * s;
+                    const R zcd = PP.zeta[kj], oo2e = PP.oo2z[kj];
+                    const R wC = R(2) * ea[kj];
+                    const R Qx = PP.Px[kj], Qy = PP.Py[kj], Qz = PP.Pz[kj];
+                    const R inv = R(1) / (zab + zcd);
+                    const R roz = zcd * inv, roe = zab * inv;
+                    const R ROU = zab * roz;
+                    const R Wx = (Px * zab + Qx * zcd) * inv;
+                    const R Wy = (Py * zab + Qy * zcd) * inv;
+                    const R Wz = (Pz * zab + Qz * zcd) * inv;
+                    const R dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                    const R T = (dx * dx + dy * dy + dz * dz) * ROU;
+                    R F[kMaxF + 1];
+                    if constexpr (f64) boys_ref_n(T, Mb, LT2, F);
+                    else boys_f32_all(T, a.boys_tab, F);
+                    const R s = X2 * PP.K[kj] * sqrt_r(inv);
+                    R fsel = F[0] * s;
 #pragma unroll
                     for (int m = 1; m <= kMaxF; ++m) fsel = (lane == m) ? F[m] * s : fsel;
                     __syncthreads();
                     if (lane <= LT2) Fl[lane] = fsel;
-                    const double PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
-                    const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
-                    const double oo2ze = 0.5 * inv;
+                    const R PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
+                    const R QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
+                    const R oo2ze = R(0.5) * inv;
                     __syncthreads();
                     for (int k = lev[0] + lane; k < lev[1]; k += kGradTeam) {
                         const uint4 nd = nodes[k];
@@ -156,19 +167,19 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                             const unsigned fl = nd.w;
                             const bool bra = (fl & 3u) == 1u;
                             const int dir = (fl >> 2) & 3;
-                            const double n1 = double((fl >> 4) & 15u), ne = double((fl >> 8) & 15u);
-                            const double pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
-                            const double wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
-                            const double c3 = n1 * (bra ? oo2z : oo2e);
-                            const double c4 = bra ? roz : roe;
-                            const double v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
-                                             c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
+                            const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
+                            const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
+                            const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
+                            const R c3 = n1 * (bra ? oo2z : oo2e);
+                            const R c4 = bra ? roz : roe;
+                            const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
+                                        c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
                             V[nd.x & 0xffffu] = v;
                         }
                     }
                     __syncthreads();
                     for (int k = lane; k < nS; k += kGradTeam) {
-                        const double wk = k < n0 ? 1.0 : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
+                        const R wk = k < n0 ? R(1) : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
                         S[k] += wk * V[tgt[k]];
                     }
                 }
@@ -179,50 +190,50 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
             if (lane < 24) {
                 const int ax = (lane % 12) >> 2, k = lane & 3;
                 const int u = lane < 12 ? atA : atC, v = lane < 12 ? atB : atD;
-                const double d = a.xyz[3 * u + ax] - a.xyz[3 * v + ax];
-                double r = 1.0;
+                const R d = R(a.xyz[3 * u + ax] - a.xyz[3 * v + ax]);
+                R r = R(1);
                 for (int e = 0; e < k; ++e) r *= d;
                 pw[lane] = r;
             }
             {
                 const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
-                double wq = 1.0;
-                if (qa == qb) wq *= 0.5;
-                if (qc == qd) wq *= 0.5;
-                if (p == q) wq *= 0.5;
+                R wq = R(1);
+                if (qa == qb) wq *= R(0.5);
+                if (qc == qd) wq *= R(0.5);
+                if (p == q) wq *= R(0.5);
                 for (int o = lane; o < NOUT; o += kGradTeam) {
                     const int id = o % CD, ic = (o / CD) % CC, ib = (o / (CD * CC)) % CB, ia = o / (CD * CC * CB);
                     const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
-                    const double Dab = a.Dp[ra + fb + ib], Dcd = a.Dp[(long long)(fc + ic) * n + fd + id];
-                    const double Dac = a.Dp[ra + fc + ic], Dbd = a.Dp[rb + fd + id];
-                    const double Dad = a.Dp[ra + fd + id], Dbc = a.Dp[rb + fc + ic];
-                    Kv[o] = wq * (4.0 * Dab * Dcd - a.hybrid * (Dac * Dbd + Dad * Dbc));
+                    const R Dab = Dp[ra + fb + ib], Dcd = Dp[(long long)(fc + ic) * n + fd + id];
+                    const R Dac = Dp[ra + fc + ic], Dbd = Dp[rb + fd + id];
+                    const R Dad = Dp[ra + fd + id], Dbc = Dp[rb + fc + ic];
+                    Kv[o] = wq * (R(4) * Dab * Dcd - hyb * (Dac * Dbd + Dad * Dbc));
                 }
             }
-            double g[9];
+            R g[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) g[k] = 0.0;
+            for (int k = 0; k < 9; ++k) g[k] = R(0);
             for (int st = 0; st < nStages; ++st) {
                 const int* rec = tab + kGhSize + st * kGradStageFields;
                 const int dir = rec[0], src = rec[1], nR1 = rec[2], nCol = rec[3], nR2 = rec[4];
                 const int* bt = tab + rec[5];
                 const int* kt = tab + rec[6];
                 const int* con = tab + rec[7];
-                const double* Ss = S + (src == 0 ? 0 : (src == 1 ? n0 : (src == 2 ? n0 + nA : n0 + nA + nB)));
-                double* o1 = V;
-                double* o2 = V + nR1 * nCol;
+                const R* Ss = S + (src == 0 ? 0 : (src == 1 ? n0 : (src == 2 ? n0 + nA : n0 + nA + nB)));
+                R* o1 = V;
+                R* o2 = V + nR1 * nCol;
                 __syncthreads();                   // pw, Kv, previous stage's readers
                 for (int idx = lane; idx < nR1 * nCol; idx += kGradTeam) {
                     const int r1 = idx / nCol, col = idx - r1 * nCol;
                     const int4 t0 = reinterpret_cast<const int4*>(bt)[2 * r1];
                     const int4 t1 = reinterpret_cast<const int4*>(bt)[2 * r1 + 1];
                     const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-                    double v = 0.0;
+                    R v = R(0);
 #pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
                         const int e = tt[u];
                         const int row = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
-                        v += double(cf) * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] *
+                        v += R(cf) * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] *
                              Ss[row * nCol + col];
                     }
                     o1[idx] = v;
@@ -233,26 +244,26 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     const int4 t0 = reinterpret_cast<const int4*>(kt)[2 * r2];
                     const int4 t1 = reinterpret_cast<const int4*>(kt)[2 * r2 + 1];
                     const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-                    double v = 0.0;
+                    R v = R(0);
 #pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
                         const int e = tt[u];
                         const int colk = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
-                        v += double(cf) * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] *
+                        v += R(cf) * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] *
                              o1[r1 * nCol + colk];
                     }
                     o2[idx] = v;
                 }
                 __syncthreads();
                 for (int o = lane; o < NOUT; o += kGradTeam) {
-                    const double K = Kv[o];
+                    const R K = Kv[o];
                     const int4 cr = reinterpret_cast<const int4*>(con)[o];
                     const int ce[3] = {cr.x, cr.y, cr.z};
 #pragma unroll
                     for (int x = 0; x < 3; ++x) {
                         const int ix = ce[x] & 0xffff;
                         if (ix != 0xffff) {
-                            const double v = double(ce[x] >> 16) * K * o2[ix];
+                            const R v = R(ce[x] >> 16) * K * o2[ix];
                             if (dir == 0) g[x] += v; else if (dir == 1) g[3 + x] += v; else g[6 + x] += v;
                         }
                     }
@@ -260,11 +271,12 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
             }
             // nine sums: over each wavefront (fixed order), then over the block's
             // wavefronts in LDS (fixed order): reproducible
+            double gs[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) g[k] = wave_sum_f64(g[k]);
+            for (int k = 0; k < 9; ++k) gs[k] = wave_sum_f64(double(g[k]));
             if ((lane & 63) == 0)
 #pragma unroll
-                for (int k = 0; k < 9; ++k) gred[(lane >> 6) * 9 + k] = g[k];
+                for (int k = 0; k < 9; ++k) gred[(lane >> 6) * 9 + k] = gs[k];
             __syncthreads();
             if (lane < 3) {
                 double gA = 0.0, gB = 0.0, gC = 0.0;
@@ -295,30 +307,38 @@ __global__ void grad_finalize(const unsigned long long* __restrict__ G, double* 
 // four wavefronts per quartet for the classes with the largest node programs
 static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1500 ? 256 : 64; }
 
-size_t grad_lds_bytes(const int* hdr) {
+// dynamic LDS of grad_kernel: the FP64 partial sums (9 x 4), then the VRR /
+// stage buffers, sets, density factors, powers and Boys values in the
+// kernel's precision
+size_t grad_lds_bytes(const int* hdr, bool fp32) {
     const int RV = hdr[kGhSlots] > hdr[kGhMaxStage] ? hdr[kGhSlots] : hdr[kGhMaxStage];
     const int nS = hdr[kGhN0] + hdr[kGhNA] + hdr[kGhNB] + hdr[kGhNC];
-    return sizeof(double) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16 + 9 * 4);
+    return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16);
 }
 
-template <int TEAM>
+template <int TEAM, typename R>
 static int launch_team(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s) {
     if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM, R>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL(grad_kernel<TEAM>, dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
+    hipLaunchKernelGGL((grad_kernel<TEAM, R>), dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // hdr: the class's table header (host copy); dtab: the same table on the device
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
-                      hipStream_t s) {
-    return grad_team_of(hdr) == 256 ? launch_team<256>(a, w, dtab, lds, grid, s)
-                                    : launch_team<64>(a, w, dtab, lds, grid, s);
+                      bool fp32, hipStream_t s) {
+    if (fp32)
+        return grad_team_of(hdr) == 256 ? launch_team<256, float>(a, w, dtab, lds, grid, s)
+                                        : launch_team<64, float>(a, w, dtab, lds, grid, s);
+    return grad_team_of(hdr) == 256 ? launch_team<256, double>(a, w, dtab, lds, grid, s)
+                                    : launch_team<64, double>(a, w, dtab, lds, grid, s);
 }
 
-#define QK_DECL(T) void launch_grad_light_##T(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
+#define QK_DECL(T)                                                                                 \
+    void launch_grad_light_##T(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);      \
+    void launch_grad_light_f32_##T(const GradArgs&, const ClassWork&, int, size_t, hipStream_t);
 QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(dsss) QK_DECL(ppps) QK_DECL(dsps)
 QK_DECL(dpss) QK_DECL(pppp) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpps) QK_DECL(ddss)
 #undef QK_DECL
@@ -329,10 +349,15 @@ static const GradLightLauncher kGradLight[kNumQuartetClasses] = {
     launch_grad_light_ppps, launch_grad_light_pppp, launch_grad_light_dsss, launch_grad_light_dsps,
     launch_grad_light_dspp, launch_grad_light_dsds, launch_grad_light_dpss, launch_grad_light_dpps, nullptr, nullptr,
     nullptr, launch_grad_light_ddss, nullptr, nullptr, nullptr, nullptr, nullptr};
+static const GradLightLauncher kGradLightF32[kNumQuartetClasses] = {
+    launch_grad_light_f32_ssss, launch_grad_light_f32_psss, launch_grad_light_f32_psps, launch_grad_light_f32_ppss,
+    launch_grad_light_f32_ppps, launch_grad_light_f32_pppp, launch_grad_light_f32_dsss, launch_grad_light_f32_dsps,
+    launch_grad_light_f32_dspp, launch_grad_light_f32_dsds, launch_grad_light_f32_dpss, launch_grad_light_f32_dpps,
+    nullptr, nullptr, nullptr, launch_grad_light_f32_ddss, nullptr, nullptr, nullptr, nullptr, nullptr};
 bool grad_class_is_light(int k) { return kGradLight[k] != nullptr; }
-int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s) {
+int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bool fp32, hipStream_t s) {
     const size_t lds = a.lds_grad ? sizeof(unsigned long long) * 3 * (size_t)a.natom : 0;
-    kGradLight[k](a, w, grid, lds, s);
+    (fp32 ? kGradLightF32 : kGradLight)[k](a, w, grid, lds, s);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

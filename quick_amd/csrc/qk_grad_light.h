@@ -51,31 +51,34 @@ QK_GRAD_LIGHT(2, 2, 0, 0, ddss)
 #undef QK_GRAD_LIGHT
 
 // Two wavefronts per SIMD when one pass's accumulators and density factors fit
-// the budget of the generator (no spills), else one.
-template <int LA, int LB, int LC, int LD>
+// the budget of the generator (no spills), else one; FP32 halves the registers.
+template <int LA, int LB, int LC, int LD, typename R>
 struct GradLightOcc {
     using S = typename GradLight<LA, LB, LC, LD>::Sets;
     static constexpr int nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-    static constexpr int value = S::kWidth + nout <= 104 ? 2 : 1;
+    static constexpr int value = (S::kWidth + nout) * (int)sizeof(R) <= 104 * 8 ? 2 : 1;
 };
 
 // primitive pairs of the bra for the gradient: the Fock slab's seven fields
 // plus the two exponents
-__device__ __forceinline__ void stage_bra_grad(const GradArgs& a, int base, int stride, int n, int lane,
-                                               double* slab) {
+template <typename R>
+__device__ __forceinline__ void stage_bra_grad(const GradArgs& a, int base, int stride, int n, int lane, R* slab) {
+    const auto& P = grad_prims<R>(a);
+    const R* ea = grad_ea<R>(a);
+    const R* eb = grad_eb<R>(a);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane < n) {
         const int k = base + lane * stride;
-        slab[lane] = a.pp.zeta[k];
-        slab[64 + lane] = a.pp.oo2z[k];
-        slab[2 * 64 + lane] = a.pp.Px[k];
-        slab[3 * 64 + lane] = a.pp.Py[k];
-        slab[4 * 64 + lane] = a.pp.Pz[k];
-        slab[5 * 64 + lane] = a.pp.K[k];
-        slab[6 * 64 + lane] = a.pp.cut[k];
-        slab[7 * 64 + lane] = a.ea[k];
-        slab[8 * 64 + lane] = a.eb[k];
+        slab[lane] = P.zeta[k];
+        slab[64 + lane] = P.oo2z[k];
+        slab[2 * 64 + lane] = P.Px[k];
+        slab[3 * 64 + lane] = P.Py[k];
+        slab[4 * 64 + lane] = P.Pz[k];
+        slab[5 * 64 + lane] = P.K[k];
+        slab[6 * 64 + lane] = P.cut[k];
+        slab[7 * 64 + lane] = ea[k];
+        slab[8 * 64 + lane] = eb[k];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -90,15 +93,19 @@ __device__ __forceinline__ void gl_add(unsigned long long* lds, unsigned long lo
     else atomicAdd(G + idx, (unsigned long long)iv);
 }
 
-template <int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) grad_light_kernel(const GradArgs a,
-                                                                                                const ClassWork w) {
+// R = double: QUICK's Boys fit (LDS coefficients); R = float (mixed
+// precision, weak quartets): the tabulated Boys expansion of boys_f32, FP32
+// arithmetic throughout, the nine sums promoted to FP64 at the accumulate.
+template <int LA, int LB, int LC, int LD, typename R>
+__global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value)) grad_light_kernel(
+    const GradArgs a, const ClassWork w) {
     using Ops = GradLight<LA, LB, LC, LD>;
+    constexpr bool f64 = sizeof(R) == 8;
     constexpr int n0 = Ops::Sets::n0, nA = Ops::Sets::nA, nB = Ops::Sets::nB, nC = Ops::Sets::nC;
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     constexpr int NOUT = NA * NB * NC * ND;
     constexpr int LT2 = LA + LB + LC + LD + 2;
-    const double X0 = 34.98683665524972;
+    const R X0 = R(34.98683665524972);
     const int n = a.n;
     const int lane = threadIdx.x & 63;
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -106,10 +113,15 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
     unsigned long long nsurv = 0, nprimq = 0;
     extern __shared__ unsigned long long s_grad[];   // 3 x natom when a.lds_grad
     __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
-    __shared__ double s_bra[4][9 * 64];
-    double* bra = s_bra[threadIdx.x >> 6];
+    __shared__ R s_bra[4][9 * 64];
+    R* bra = s_bra[threadIdx.x >> 6];
     const bool use_lds = a.lds_grad != 0;
-    for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    const auto& PP = grad_prims<R>(a);
+    const R* ea = grad_ea<R>(a);
+    const R* Dp = grad_density<R>(a);
+    const R rcut = R(a.cutoff), hyb = R(a.hybrid);
+    if constexpr (f64)
+        for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
     if (use_lds)
         for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x) s_grad[k] = 0ull;
     __syncthreads();
@@ -131,19 +143,19 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
             active = ct > a.cutoff && ct * dn > a.cutoff;
         }
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra_grad(a, pb0, pbs, nbp, lane, bra);
+        stage_bra_grad<R>(a, pb0, pbs, nbp, lane, bra);
         if (!active) continue;
         ++nsurv;
         const int Mb = a.q.qfinal[qa] + a.q.qfinal[qb] + a.q.qfinal[qc] + a.q.qfinal[qd] + 2;
-        const double Ax = a.xyz[3 * atA], Ay = a.xyz[3 * atA + 1], Az = a.xyz[3 * atA + 2];
-        const double Cx = a.xyz[3 * atC], Cy = a.xyz[3 * atC + 1], Cz = a.xyz[3 * atC + 2];
+        const R Ax = R(a.xyz[3 * atA]), Ay = R(a.xyz[3 * atA + 1]), Az = R(a.xyz[3 * atA + 2]);
+        const R Cx = R(a.xyz[3 * atC]), Cy = R(a.xyz[3 * atC + 1]), Cz = R(a.xyz[3 * atC + 2]);
         // density factors K = w (4 D_ab D_cd - h (D_ac D_bd + D_ad D_bc)), cons folded into D
         const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
-        double wq = 1.0;
-        if (qa == qb) wq *= 0.5;
-        if (qc == qd) wq *= 0.5;
-        if (p == q) wq *= 0.5;
-        double K[NOUT];
+        R wq = R(1);
+        if (qa == qb) wq *= R(0.5);
+        if (qc == qd) wq *= R(0.5);
+        if (p == q) wq *= R(0.5);
+        R K[NOUT];
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -153,66 +165,68 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
 #pragma unroll
                     for (int id = 0; id < ND; ++id) {
                         const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
-                        const double Dab = a.Dp[ra + fb + ib], Dcd = a.Dp[(long long)(fc + ic) * n + fd + id];
-                        const double Dac = a.Dp[ra + fc + ic], Dbd = a.Dp[rb + fd + id];
-                        const double Dad = a.Dp[ra + fd + id], Dbc = a.Dp[rb + fc + ic];
+                        const R Dab = Dp[ra + fb + ib], Dcd = Dp[(long long)(fc + ic) * n + fd + id];
+                        const R Dac = Dp[ra + fc + ic], Dbd = Dp[rb + fd + id];
+                        const R Dad = Dp[ra + fd + id], Dbc = Dp[rb + fc + ic];
                         K[((ia * NB + ib) * NC + ic) * ND + id] =
-                            wq * (4.0 * Dab * Dcd - a.hybrid * (Dac * Dbd + Dad * Dbc));
+                            wq * (R(4) * Dab * Dcd - hyb * (Dac * Dbd + Dad * Dbc));
                     }
-        double g[9];
+        R g[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) g[k] = 0.0;
-        const double Bx = a.xyz[3 * atB], By = a.xyz[3 * atB + 1], Bz = a.xyz[3 * atB + 2];
-        const double Dx = a.xyz[3 * atD], Dy = a.xyz[3 * atD + 1], Dz = a.xyz[3 * atD + 2];
+        for (int k = 0; k < 9; ++k) g[k] = R(0);
+        const R Bx = R(a.xyz[3 * atB]), By = R(a.xyz[3 * atB + 1]), Bz = R(a.xyz[3 * atB + 2]);
+        const R Dx = R(a.xyz[3 * atD]), Dy = R(a.xyz[3 * atD + 1]), Dz = R(a.xyz[3 * atD + 2]);
+        const R rdn = R(dn);
         const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
         // one pass over the primitive quartets per group of sets (the VRR is
         // recomputed; the sets of one pass are the only accumulators live)
         auto pass = [&](auto mtag) {
             constexpr int M = decltype(mtag)::value;
-            double acc0[(M & 1) ? n0 : 1], accA[(M & 2) ? nA : 1], accB[(M & 4) ? nB : 1], accC[(M & 8) ? nC : 1];
+            R acc0[(M & 1) ? n0 : 1], accA[(M & 2) ? nA : 1], accB[(M & 4) ? nB : 1], accC[(M & 8) ? nC : 1];
 #pragma unroll
-            for (int k = 0; k < ((M & 1) ? n0 : 1); ++k) acc0[k] = 0.0;
+            for (int k = 0; k < ((M & 1) ? n0 : 1); ++k) acc0[k] = R(0);
 #pragma unroll
-            for (int k = 0; k < ((M & 2) ? nA : 1); ++k) accA[k] = 0.0;
+            for (int k = 0; k < ((M & 2) ? nA : 1); ++k) accA[k] = R(0);
 #pragma unroll
-            for (int k = 0; k < ((M & 4) ? nB : 1); ++k) accB[k] = 0.0;
+            for (int k = 0; k < ((M & 4) ? nB : 1); ++k) accB[k] = R(0);
 #pragma unroll
-            for (int k = 0; k < ((M & 8) ? nC : 1); ++k) accC[k] = 0.0;
+            for (int k = 0; k < ((M & 8) ? nC : 1); ++k) accC[k] = R(0);
             {
                 const int ni = nbp;   // <= 64 primitive pairs per Qshell pair (checked on the host)
                 for (int j = 0; j < nkp; ++j) {
                     const int kj = pk0 + j * pks;
-                    const double zcd = a.pp.zeta[kj], oo2e = a.pp.oo2z[kj];
-                    const double Qx = a.pp.Px[kj], Qy = a.pp.Py[kj], Qz = a.pp.Pz[kj];
-                    const double Kcd = a.pp.K[kj], ccd = a.pp.cut[kj], wC = 2.0 * a.ea[kj];
-                    const double QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
+                    const R zcd = PP.zeta[kj], oo2e = PP.oo2z[kj];
+                    const R Qx = PP.Px[kj], Qy = PP.Py[kj], Qz = PP.Pz[kj];
+                    const R Kcd = PP.K[kj], ccd = PP.cut[kj], wC = R(2) * ea[kj];
+                    const R QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz;
                     for (int i = 0; i < ni; ++i) {
-                        if (!((dn * bra[6 * 64 + i]) * ccd > a.cutoff)) continue;
+                        if (!((rdn * bra[6 * 64 + i]) * ccd > rcut)) continue;
                         if (M == Ops::Sets::kMask[0]) ++nprimq;
-                        const double zab = bra[i], oo2z = bra[64 + i];
-                        const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                        const double inv = 1.0 / (zab + zcd);
-                        const double roz = zcd * inv, roe = zab * inv;
-                        const double ROU = zab * roz;
-                        const double Wx = (Px * zab + Qx * zcd) * inv;
-                        const double Wy = (Py * zab + Qy * zcd) * inv;
-                        const double Wz = (Pz * zab + Qz * zcd) * inv;
-                        const double dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
-                        const double T = (dx * dx + dy * dy + dz * dz) * ROU;
-                        double F[LT2 + 1];
-                        boys_fit<LT2>(T, Mb, s_boys, F);
-                        const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
+                        const R zab = bra[i], oo2z = bra[64 + i];
+                        const R Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
+                        const R inv = R(1) / (zab + zcd);
+                        const R roz = zcd * inv, roe = zab * inv;
+                        const R ROU = zab * roz;
+                        const R Wx = (Px * zab + Qx * zcd) * inv;
+                        const R Wy = (Py * zab + Qy * zcd) * inv;
+                        const R Wz = (Pz * zab + Qz * zcd) * inv;
+                        const R dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
+                        const R T = (dx * dx + dy * dy + dz * dz) * ROU;
+                        R F[LT2 + 1];
+                        if constexpr (f64) boys_fit<LT2>(T, Mb, s_boys, F);
+                        else boys_f32<LT2>(T, a.boys_tab, F);
+                        const R s = X0 * bra[5 * 64 + i] * Kcd * sqrt_r(inv);
 #pragma unroll
                         for (int m = 0; m <= LT2; ++m) F[m] *= s;
-                        Ops::template vrg<M, double>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx,
-                                                     QCy, QCz, Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, 0.5 * inv,
-                                                     roz, roe, 2.0 * bra[7 * 64 + i], 2.0 * bra[8 * 64 + i], wC,
-                                                     acc0, accA, accB, accC);
+                        Ops::template vrg<M, R>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx,
+                                                QCy, QCz, Wx - Qx, Wy - Qy, Wz - Qz, oo2z, oo2e, R(0.5) * inv,
+                                                roz, roe, R(2) * bra[7 * 64 + i], R(2) * bra[8 * 64 + i], wC,
+                                                acc0, accA, accB, accC);
                     }
                 }
             }
-            Ops::template gcon<M, double>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
-                                          Cz - Dz, K, g);
+            Ops::template gcon<M, R>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
+                                     Cz - Dz, K, g);
         };
         pass(std::integral_constant<int, Ops::Sets::kMask[0]>{});
         if constexpr (Ops::Sets::kPasses > 1) {
@@ -226,10 +240,11 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD>::value)) gr
         }
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            gl_add(s_grad, a.G, use_lds, 3 * atA + x, g[x]);
-            gl_add(s_grad, a.G, use_lds, 3 * atB + x, g[3 + x]);
-            gl_add(s_grad, a.G, use_lds, 3 * atC + x, g[6 + x]);
-            gl_add(s_grad, a.G, use_lds, 3 * atD + x, -(g[x] + g[3 + x] + g[6 + x]));
+            const double gA = double(g[x]), gB = double(g[3 + x]), gC = double(g[6 + x]);
+            gl_add(s_grad, a.G, use_lds, 3 * atA + x, gA);
+            gl_add(s_grad, a.G, use_lds, 3 * atB + x, gB);
+            gl_add(s_grad, a.G, use_lds, 3 * atC + x, gC);
+            gl_add(s_grad, a.G, use_lds, 3 * atD + x, -(gA + gB + gC));
         }
     }
     if (nsurv) atomicAdd(a.counters + 0, nsurv);

@@ -29,12 +29,12 @@ size_t fock_class_scratch_f32(int k);
 void launch_team_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_team_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, hipStream_t s);
 void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
-size_t grad_lds_bytes(const int* hdr);
+size_t grad_lds_bytes(const int* hdr, bool fp32);
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
-                      hipStream_t s);
+                      bool fp32, hipStream_t s);
 void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
 bool grad_class_is_light(int k);
-int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, hipStream_t s);
+int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bool fp32, hipStream_t s);
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
@@ -185,6 +185,7 @@ struct qk_ctx {
     DBuf<float> dkf_zeta, dkf_oo2z, dkf_Px, dkf_Py, dkf_Pz, dkf_K, dkf_cut;   // FP32 copy
     DBuf<float> d_boys_tab, d_Dpf;
     DBuf<double> dk_ea, dk_eb;   // exponents, primitive-major like dk_*
+    DBuf<float> dkf_ea, dkf_eb;   // FP32 copy (mixed-precision gradient)
 
     // prepared work lists and the options they were built for
     bool prepared = false;
@@ -199,9 +200,10 @@ struct qk_ctx {
 
     // gradient: per-class tables on the device, work lists, accumulators
     DBuf<int> d_gtab[kNumQuartetClasses];
-    std::vector<Tile> gtiles[kNumQuartetClasses];
-    DBuf<Tile> d_gtiles[kNumQuartetClasses];
-    double grad_prep_cutoff = -1;
+    std::vector<Tile> gtiles[kNumQuartetClasses], gtiles32[kNumQuartetClasses];   // FP64 / FP32 work
+    DBuf<Tile> d_gtiles[kNumQuartetClasses], d_gtiles32[kNumQuartetClasses];
+    double grad_prep_cutoff = -1, grad_prep_thr = -1;
+    int grad_prep_precision = -1;
     bool grad_prepared = false;
     DBuf<unsigned long long> d_gacc, d_gcounters;
     long long gcls_quartets[kNumQuartetClasses] = {};
@@ -230,8 +232,9 @@ struct qk_ctx {
     // optional per-class kernel timing (HIP events on the launch stream)
     bool timing = false;
     // (slot k: the FP64 launch of class k; slot kNumQuartetClasses + k: its FP32
-    // launch; slot 2 kNumQuartetClasses + k: its gradient launch)
-    static constexpr int kSlots = 3 * kNumQuartetClasses;
+    // launch; slot 2 kNumQuartetClasses + k: its gradient launch; slot
+    // 3 kNumQuartetClasses + k: its FP32 gradient launch)
+    static constexpr int kSlots = 4 * kNumQuartetClasses;
     hipEvent_t ev_t0[kSlots] = {}, ev_t1[kSlots] = {};
     bool cls_timed[kSlots] = {};
     double cls_ms_sum[kSlots] = {};
@@ -612,7 +615,8 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
         if (c->dkf_zeta.upload(f32(kz)) || c->dkf_oo2z.upload(f32(ko)) || c->dkf_Px.upload(f32(kx)) ||
             c->dkf_Py.upload(f32(ky)) || c->dkf_Pz.upload(f32(kzz)) || c->dkf_K.upload(f32(kK)) ||
-            c->dkf_cut.upload(f32(kc)) || c->d_Dpf.alloc((size_t)c->nbasis * c->nbasis))
+            c->dkf_cut.upload(f32(kc)) || c->dkf_ea.upload(f32(kea)) || c->dkf_eb.upload(f32(keb)) ||
+            c->d_Dpf.alloc((size_t)c->nbasis * c->nbasis))
             return -1;
         if (!c->d_boys_tab.p) {
             std::vector<float> tab;
@@ -987,9 +991,15 @@ extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_o
 
 // Work lists of the gradient: the prepared pair groups, every bra with the
 // prefix of kets passing Y_ij*Y_kl > cutoff, in runs of <= chunk quartets (a
-// wavefront works through a run one quartet after another).
-static int grad_prepare(qk_ctx* c, double cutoff) {
-    if (c->grad_prepared && c->grad_prep_cutoff == cutoff) return 0;
+// wavefront works through a run one quartet after another).  Mixed precision:
+// the kets with Y_ij*Y_kl below the FP32 threshold (a suffix of the prefix, as
+// kets are sorted by Y) go to the FP32 list.
+static int grad_prepare(qk_ctx* c, const qk_opts* o) {
+    const bool mixed = o->precision == QK_MIXED;
+    const double cutoff = o->tau;
+    if (c->grad_prepared && c->grad_prep_cutoff == cutoff && c->grad_prep_precision == o->precision &&
+        (!mixed || c->grad_prep_thr == o->fp32_threshold))
+        return 0;
     (void)hipSetDevice(c->device);
     for (int k = 0; k < kNumQuartetClasses; ++k) {
         if (!c->d_gtab[k].p) {
@@ -1003,7 +1013,9 @@ static int grad_prepare(qk_ctx* c, double cutoff) {
         // table-driven kernel: one quartet at a time per wavefront
         const int chunk = grad_class_is_light(k) ? 64 : (LT <= 3 ? 16 : (LT <= 5 ? 4 : 1));
         std::vector<Tile>& T = c->gtiles[k];
+        std::vector<Tile>& T32 = c->gtiles32[k];
         T.clear();
+        T32.clear();
         const size_t ng = c->grp_cls.size();
         for (size_t gb = 0; gb < ng; ++gb) {
             if (c->grp_cls[gb] != cb) continue;
@@ -1020,15 +1032,29 @@ static int grad_prepare(qk_ctx* c, double cutoff) {
                         const int mid = (lo + hi) >> 1;
                         if (yp * c->pair_Y[s2 + mid] > cutoff) lo = mid + 1; else hi = mid;
                     }
-                    for (int q = same ? i : 0; q < lo; q += chunk)
-                        T.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
+                    const int q_begin = same ? i : 0;
+                    int lo64 = lo;
+                    if (mixed) {
+                        int a0 = 0, a1 = lo;
+                        while (a0 < a1) {
+                            const int mid = (a0 + a1) >> 1;
+                            if (yp * c->pair_Y[s2 + mid] >= o->fp32_threshold) a0 = mid + 1; else a1 = mid;
+                        }
+                        lo64 = std::max(a0, q_begin);
+                    }
+                    for (int q = q_begin; q < lo64; q += chunk)
+                        T.push_back({s1 + i, s2 + q, std::min(chunk, lo64 - q), 0});
+                    for (int q = lo64; q < lo; q += chunk)
+                        T32.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
                 }
             }
         }
-        if (c->d_gtiles[k].upload(T)) return -1;
+        if (c->d_gtiles[k].upload(T) || c->d_gtiles32[k].upload(T32)) return -1;
     }
-    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(2 * kNumQuartetClasses)) return -1;
+    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses)) return -1;
     c->grad_prep_cutoff = cutoff;
+    c->grad_prep_precision = o->precision;
+    c->grad_prep_thr = o->fp32_threshold;
     c->grad_prepared = true;
     return 0;
 }
@@ -1036,18 +1062,19 @@ static int grad_prepare(qk_ctx* c, double cutoff) {
 // dgrad <- dgrad + dE2/dR for density dD (cutmatrix from dD unless given)
 static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_null, double* dgrad,
                         const qk_opts* o, hipStream_t s) {
-    if (o->precision != QK_FP64) { set_err("the gradient is FP64 only"); return -4; }
     if (prepare(c, o)) return -1;
-    if (grad_prepare(c, o->tau)) return -1;
+    if (grad_prepare(c, o)) return -1;
     if (fold_timing(c)) return -1;
+    const bool mixed = o->precision == QK_MIXED;
+    if (mixed && !c->d_boys_tab.p) { set_err("mixed precision: Boys table missing"); return -1; }
     const int n = c->nbasis, ns = c->nshell;
-    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, nullptr, n, s);
+    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, mixed ? c->d_Dpf.p : nullptr, n, s);
     if (dcutmat_or_null)
         HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
     launch_fill_u64(c->d_gacc.p, 3LL * c->natom, s);
-    launch_fill_u64(c->d_gcounters.p, 2LL * kNumQuartetClasses, s);
+    launch_fill_u64(c->d_gcounters.p, 4LL * kNumQuartetClasses, s);
     HIPCHK(hipGetLastError());
     GradArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
@@ -1065,28 +1092,37 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     a.hybrid = o->hybrid;
     a.natom = c->natom;
     a.lds_grad = 3 * c->natom <= 8192 ? 1 : 0;   // per-block LDS copy of the gradient up to 64 KB
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
-        if (c->gtiles[k].empty()) continue;
-        ClassWork w{c->d_gtiles[k].p, (long long)c->gtiles[k].size()};
-        a.counters = c->d_gcounters.p + 2 * k;
-        const int slot = 2 * kNumQuartetClasses + k;
-        if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
-        if (grad_class_is_light(k)) {
-            const int grid = (int)std::min<long long>(((long long)c->gtiles[k].size() + 3) / 4, 256LL * 8);
-            if (launch_grad_light(k, a, w, grid, s)) {
-                set_err("gradient kernel launch failed for class %d", k);
-                return -1;
+    a.ppf = {c->dkf_zeta.p, c->dkf_oo2z.p, c->dkf_Px.p, c->dkf_Py.p, c->dkf_Pz.p, c->dkf_K.p, c->dkf_cut.p};
+    a.eaf = c->dkf_ea.p;
+    a.ebf = c->dkf_eb.p;
+    a.Dpf = c->d_Dpf.p;
+    a.boys_tab = c->d_boys_tab.p;
+    for (int pass = 0; pass < (mixed ? 2 : 1); ++pass) {
+        const bool fp32 = pass == 1;
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            const std::vector<Tile>& T = fp32 ? c->gtiles32[k] : c->gtiles[k];
+            if (T.empty()) continue;
+            ClassWork w{(fp32 ? c->d_gtiles32[k] : c->d_gtiles[k]).p, (long long)T.size()};
+            a.counters = c->d_gcounters.p + 2 * (fp32 ? kNumQuartetClasses + k : k);
+            const int slot = (fp32 ? 3 : 2) * kNumQuartetClasses + k;
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
+            if (grad_class_is_light(k)) {
+                const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
+                if (launch_grad_light(k, a, w, grid, fp32, s)) {
+                    set_err("gradient kernel launch failed for class %d", k);
+                    return -1;
+                }
+            } else {
+                const size_t lds = grad_lds_bytes(kGradBlobs[k], fp32);
+                const int grid = (int)std::min<long long>((long long)T.size(), 256LL * 16);
+                if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, s)) {
+                    set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
+                    return -1;
+                }
             }
-        } else {
-        const size_t lds = grad_lds_bytes(kGradBlobs[k]);
-        const int grid = (int)std::min<long long>((long long)c->gtiles[k].size(), 256LL * 16);
-        if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, s)) {
-            set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
-            return -1;
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
+            c->cls_timed[slot] = c->timing;
         }
-        }
-        if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
-        c->cls_timed[slot] = c->timing;
     }
     launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, s);
     HIPCHK(hipGetLastError());
@@ -1116,6 +1152,7 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
 }
 
 // surviving quartets and primitive quartets of class k in the last gradient
+// (FP64 part), and its kernel time
 extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* np, float* ms) {
     if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
     (void)hipSetDevice(c->device);
@@ -1127,6 +1164,20 @@ extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* n
     *nq = (long long)h[0];
     *np = (long long)h[1];
     return slot_ms(c, 2 * kNumQuartetClasses + k, ms);
+}
+
+// the same for the FP32 part of a mixed-precision gradient
+extern "C" int qk_grad_class_stats_fp32(qk_ctx* c, int k, long long* nq, long long* np, float* ms) {
+    if (!c || k < 0 || k >= kNumQuartetClasses) { set_err("bad arguments"); return -1; }
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipDeviceSynchronize());
+    *ms = 0.0f;
+    if (!c->d_gcounters.p) { *nq = *np = 0; return 0; }
+    unsigned long long h[2];
+    HIPCHK(hipMemcpy(h, c->d_gcounters.p + 2 * (kNumQuartetClasses + k), sizeof(h), hipMemcpyDeviceToHost));
+    *nq = (long long)h[0];
+    *np = (long long)h[1];
+    return slot_ms(c, 3 * kNumQuartetClasses + k, ms);
 }
 
 // ---------------------------------------------------------------------------

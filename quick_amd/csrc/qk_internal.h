@@ -122,6 +122,12 @@ struct GradArgs {
     double hybrid;
     int natom;
     int lds_grad;       // light-class kernels: accumulate per block in LDS (3 x natom) first
+    // FP32 copies for the mixed-precision kernels (set when precision is mixed)
+    DevPrimsF ppf;
+    const float* eaf;
+    const float* ebf;
+    const float* Dpf;
+    const float* boys_tab;   // kBoysTabRows x kBoysTabM
 };
 
 struct FockArgs {

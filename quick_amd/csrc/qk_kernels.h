@@ -472,6 +472,35 @@ __device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab,
     }
 }
 
+// the primitive-pair data, exponents and density in the kernel's precision
+template <typename R>
+__device__ __forceinline__ const auto& grad_prims(const GradArgs& a) {
+    if constexpr (sizeof(R) == 8) return a.pp; else return a.ppf;
+}
+template <typename R>
+__device__ __forceinline__ const R* grad_ea(const GradArgs& a) {
+    if constexpr (sizeof(R) == 8) return a.ea; else return a.eaf;
+}
+template <typename R>
+__device__ __forceinline__ const R* grad_eb(const GradArgs& a) {
+    if constexpr (sizeof(R) == 8) return a.eb; else return a.ebf;
+}
+template <typename R>
+__device__ __forceinline__ const R* grad_density(const GradArgs& a) {
+    if constexpr (sizeof(R) == 8) return a.Dp; else return a.Dpf;
+}
+__device__ __forceinline__ double sqrt_r(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_r(float x) { return sqrtf(x); }
+
+// F_0..F_kMaxGradF(T) in FP32 for the gradient's table-driven kernel: the
+// expansion of boys_f32 at the highest order the table supports for it, then
+// downward recursion (stable), so every order is valid whatever the class
+constexpr int kMaxGradF = 10;
+static_assert(kMaxGradF + 4 < kBoysTabM, "Boys table too narrow for the gradient");
+__device__ __forceinline__ void boys_f32_all(float T, const float* __restrict__ tab, float* F) {
+    boys_f32<kMaxGradF>(T, tab, F);
+}
+
 __device__ __forceinline__ void fx_add_f(unsigned long long* X, long long idx, float v) {
     const long long iv = __double2ll_rn(double(v) * 0x1p48);
     if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);

@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
                                      C.POINTER(C.c_double), C.POINTER(C.c_float)]
         L.qk_grad_class_stats.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_float)]
+        L.qk_grad_class_stats_fp32.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                               C.POINTER(C.c_float)]
         L.qk_class_stats_fp32.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_double), C.POINTER(C.c_float)]
         _lib = L
@@ -187,16 +189,18 @@ class FockEngine:
                "qk_grad_host")
         return g.reshape(3, self.natom, order="F")
 
-    def grad_class_stats(self):
-        """[(tag, quartets, prim_quartets, ms)] of the last gradient (ms when timing is on)."""
+    def grad_class_stats(self, fp32: bool = False):
+        """[(tag, quartets, prim_quartets, ms)] of the last gradient (ms when timing
+        is on); fp32: the FP32 part of a mixed-precision gradient."""
         out = []
         L = lib()
+        fn = L.qk_grad_class_stats_fp32 if fp32 else L.qk_grad_class_stats
         l4 = (C.c_int * 4)()
         w = C.c_longlong()
         nq, npq, ms = C.c_longlong(), C.c_longlong(), C.c_float()
         for k in range(L.qk_num_classes()):
             _check(L.qk_class_info(self.h, k, l4, C.byref(w)), "qk_class_info")
-            _check(L.qk_grad_class_stats(self.h, k, C.byref(nq), C.byref(npq), C.byref(ms)), "qk_grad_class_stats")
+            _check(fn(self.h, k, C.byref(nq), C.byref(npq), C.byref(ms)), "qk_grad_class_stats")
             out.append(("".join("spd"[x] for x in l4), nq.value, npq.value, ms.value))
         return out
 

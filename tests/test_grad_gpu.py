@@ -116,3 +116,72 @@ def test_icosane_grad_central_differences():
               flush=True)
         worst = max(worst, abs(g[ax, atom] - fd))
     assert worst <= 1e-5
+
+
+# ---------------------------------------------------------------------------
+# mixed precision: quartets with Schwarz bound below the threshold differentiated
+# in FP32, accumulated in FP64 fixed point
+
+TOL_G_MIXED = 1e-6   # Eh/bohr, mixed against FP64
+
+
+def _fp32_share(eng):
+    q64 = sum(c[1] for c in eng.grad_class_stats())
+    q32 = sum(c[1] for c in eng.grad_class_stats(fp32=True))
+    return q32 / max(q64 + q32, 1), q64 + q32
+
+
+def test_grad_mixed_threshold_zero_is_fp64(golden):
+    """threshold 0: every quartet stays FP64, bitwise the FP64 gradient"""
+    from quick_amd.fock import QK_MIXED
+    _banner("grad mixed threshold 0 water4")
+    fx = golden("water4")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    g64 = eng.grad(fx.D)
+    g0 = eng.grad(fx.D, make_opts(precision=QK_MIXED, fp32_threshold=0.0))
+    assert _fp32_share(eng)[0] == 0.0
+    assert np.array_equal(g0, g64)
+
+
+@pytest.mark.parametrize("name", ["water2", "water4"])
+def test_grad_mixed_against_reference(golden, name):
+    """an FP32 share of most quartets (threshold 1e-2) stays within 1e-6 of the
+    reference gradient, and the same quartets are counted as in FP64"""
+    from quick_amd.fock import QK_MIXED
+    _banner(f"grad mixed {name}")
+    fx = golden(name)
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    g64 = eng.grad(fx.D)
+    n64 = sum(c[1] for c in eng.grad_class_stats())
+    gm = eng.grad(fx.D, make_opts(precision=QK_MIXED, fp32_threshold=1e-2))
+    share, nq = _fp32_share(eng)
+    d = np.abs(gm - fx.grad4c).max()
+    print(f"{name}: fp32 share {share:.3f} max|dgrad| vs reference {d:.2e}, vs FP64 {np.abs(gm - g64).max():.2e}")
+    assert share > 0.3
+    assert nq == n64
+    assert d <= TOL_G_MIXED
+    assert np.abs(gm.sum(axis=1)).max() <= 1e-9   # D = -(A+B+C) holds per quartet in any precision
+    # reproducible
+    assert np.array_equal(gm, eng.grad(fx.D, make_opts(precision=QK_MIXED, fp32_threshold=1e-2)))
+
+
+@pytest.mark.parametrize("system", ["water32", "icosane"])
+def test_grad_mixed_full_size(system):
+    """(H2O)32 and n-icosane: the mixed gradient (threshold 1e-3) against the
+    FP64 gradient, max error <= 1e-6 Eh/bohr"""
+    from quick_amd.fock import QK_MIXED
+    _banner(f"grad mixed {system}")
+    syms, xyz = CONFIGS[system]()
+    b = build_basis(syms, xyz)
+    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, 7)
+    eng = FockEngine(b)
+    eng.compute_schwarz()
+    g64 = eng.grad(D)
+    gm = eng.grad(D, make_opts(precision=QK_MIXED, fp32_threshold=1e-3))
+    share, _ = _fp32_share(eng)
+    d = np.abs(gm - g64).max()
+    print(f"{system}: fp32 share {share:.3f} max|dgrad| {d:.2e} (max|grad| {np.abs(g64).max():.3f})", flush=True)
+    assert share > 0.5
+    assert d <= TOL_G_MIXED

@@ -4,7 +4,8 @@ The straight-line code is plain templated C++; it is compiled here for the host
 with g++ (the device qualifiers defined away), driven over the primitive
 quartets of one shell quartet on four distinct oxygen centres, and compared
 with the table-driven gradient program of the same class (checked against
-finite differences in test_grad_tables.py).
+finite differences in test_grad_tables.py); its FP32 instantiation, used by the
+mixed-precision kernels, is held to 2e-5 of the largest derivative.
 """
 import ctypes
 import math
@@ -34,12 +35,12 @@ def shim(tmp_path_factory):
 // One pass over the primitive quartets for the contracted sets named by mask M
 // (bit 0: S0, 1: SA, 2: SB, 3: SC), then its contraction into g; the device
 // kernel runs the passes of GradSets_<class>::kMask in the same way.
-template <class S, int M, class VRG, class GCON>
-static void one_pass(int np, const double* F, int nf, const double* geo, const double* wts, const double* ABCD,
-                     const double* K, double* g, VRG vrg, GCON gcon) {
-    double a0[S::n0] = {}, aA[S::nA] = {}, aB[S::nB] = {}, aC[S::nC] = {};
+template <class S, int M, class R, class VRG, class GCON>
+static void one_pass(int np, const R* F, int nf, const R* geo, const R* wts, const R* ABCD,
+                     const R* K, R* g, VRG vrg, GCON gcon) {
+    R a0[S::n0] = {}, aA[S::nA] = {}, aB[S::nB] = {}, aC[S::nC] = {};
     for (int p = 0; p < np; ++p) {
-        const double* G = geo + 17 * p;
+        const R* G = geo + 17 * p;
         vrg(F + nf * p, G[0], G[1], G[2], G[3], G[4], G[5], G[6], G[7], G[8], G[9], G[10], G[11],
             G[12], G[13], G[14], G[15], G[16], wts[3 * p], wts[3 * p + 1], wts[3 * p + 2], a0, aA, aB, aC);
     }
@@ -47,14 +48,15 @@ static void one_pass(int np, const double* F, int nf, const double* geo, const d
 }"""]
     for cls in gen_grad.LIGHT_CLASSES:
         t = gen_eri.tag(cls)
-        passes = "\n".join(
-            f"    if constexpr (S::kPasses > {k}) one_pass<S, S::kMask[{k}]>(np, F, nf, geo, wts, ABCD, K, g, "
-            f"vrg_{t}<S::kMask[{k}], double>, gcon_{t}<S::kMask[{k}], double>);" for k in range(4))
-        src.append(f"""extern "C" void run_{t}(int np, const double* F, const double* geo, const double* wts,
-                        const double* ABCD, const double* K, double* g) {{
+        for R, suf in (("double", ""), ("float", "_f32")):
+            passes = "\n".join(
+                f"    if constexpr (S::kPasses > {k}) one_pass<S, S::kMask[{k}], {R}>(np, F, nf, geo, wts, ABCD, K, g, "
+                f"vrg_{t}<S::kMask[{k}], {R}>, gcon_{t}<S::kMask[{k}], {R}>);" for k in range(4))
+            src.append(f"""extern "C" void run{suf}_{t}(int np, const {R}* F, const {R}* geo, const {R}* wts,
+                        const {R}* ABCD, const {R}* K, {R}* g) {{
     using S = GradSets_{t};
     const int nf = {sum(cls) + 3};
-    for (int k = 0; k < 9; ++k) g[k] = 0.0;
+    for (int k = 0; k < 9; ++k) g[k] = 0;
 {passes}
 }}""")
     (d / "shim.cpp").write_text("\n".join(src) + "\n")
@@ -111,8 +113,17 @@ def test_light_gradient_code_matches_table_program(shim, o4, cls):
     g = np.zeros(9)
     dp = ctypes.POINTER(ctypes.c_double)
     p = lambda arr: arr.ctypes.data_as(dp)
-    getattr(shim, f"run_{gen_eri.tag(cls)}")(len(Fs), p(F), p(G), p(Wt), p(ABCD), p(np.ascontiguousarray(K.ravel())),
-                                             p(g))
+    Kf = np.ascontiguousarray(K.ravel())
+    getattr(shim, f"run_{gen_eri.tag(cls)}")(len(Fs), p(F), p(G), p(Wt), p(ABCD), p(Kf), p(g))
     scale = np.abs(ref).max()
     assert scale > 1e-10
     assert np.abs(g - ref).max() <= 1e-11 * max(1.0, scale), (cls, g, ref)
+    # the same code instantiated in FP32 (the mixed-precision kernels)
+    fp = ctypes.POINTER(ctypes.c_float)
+    q = lambda arr: arr.ctypes.data_as(fp)
+    f32 = lambda arr: np.ascontiguousarray(arr, dtype=np.float32)
+    F32, G32, W32, ABCD32, K32 = f32(F), f32(G), f32(Wt), f32(ABCD), f32(Kf)
+    g32 = np.zeros(9, dtype=np.float32)
+    getattr(shim, f"run_f32_{gen_eri.tag(cls)}")(len(Fs), q(F32), q(G32), q(W32), q(ABCD32), q(K32), q(g32))
+    err = np.abs(g32.astype(np.float64) - ref).max()
+    assert err <= 2e-5 * scale, (cls, err, scale)
