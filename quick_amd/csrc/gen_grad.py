@@ -217,7 +217,11 @@ def class_blob(cls):
                         con.append(0xffff)   # pad each quartet's record to four entries
         off_con = put(con)
         assert nR1 == len(o1) * n2 and nR2 == n3 * n4
-        stage_recs.append([dirn, src, nR1, ncol, nR2, off_bra, off_ket, off_con])
+        # the most terms any row of the stage's bra / ket table has (the rest of
+        # a row's MAX_TERMS slots are padding), stored above the direction
+        nb = max(sum(1 for t in row if t[2] != 0) for row in bra)
+        nk = max(sum(1 for t in row if t[2] != 0) for row in ket)
+        stage_recs.append([dirn | (nb << 8) | (nk << 16), src, nR1, ncol, nR2, off_bra, off_ket, off_con])
     n_sets = [len(E) * len(F) for E, F in sets]
     max_stage = max(rec[2] * rec[3] + rec[2] * rec[4] for rec in stage_recs)
     hdr = [len(nodes), len(lev) - 1, nslots, la + lb + lc + ld,

@@ -87,8 +87,10 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
     R* V = reinterpret_cast<R*>(lds + 36);   // VRR slots; after the primitive loop the stage buffers
     R* S = V + RV;                // S0 | SA | SB | SC
     R* Kv = S + nS;               // density factor per function quartet
-    R* pw = Kv + NOUT;            // pw[axis*4+k] = AB_axis^k, pw[12+axis*4+k] = CD_axis^k
-    R* Fl = pw + 24;
+    // pwc[c] = AB_x^(c&3) AB_y^((c>>2)&3) AB_z^((c>>4)&3) for the power code c of
+    // a bra HRR term; pwc[64 + c] the same in C-D for the ket HRR
+    R* pwc = Kv + NOUT;
+    R* Fl = pwc + 128;
     const auto& PP = grad_prims<R>(a);
     const R* ea = grad_ea<R>(a);
     const R* eb = grad_eb<R>(a);
@@ -162,38 +164,67 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     }
                     for (int L = 1; L < nLevels; ++L) {
                         __syncthreads();
-                        for (int k = lev[L] + lane; k < lev[L + 1]; k += kGradTeam) {
-                            const uint4 nd = nodes[k];
-                            const unsigned fl = nd.w;
-                            const bool bra = (fl & 3u) == 1u;
-                            const int dir = (fl >> 2) & 3;
-                            const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
-                            const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
-                            const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
-                            const R c3 = n1 * (bra ? oo2z : oo2e);
-                            const R c4 = bra ? roz : roe;
-                            const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
-                                        c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
-                            V[nd.x & 0xffffu] = v;
+                        const int k1 = lev[L + 1];
+                        // the descriptors of up to four of the lane's nodes are
+                        // loaded before any is evaluated (one memory latency)
+                        for (int kb = lev[L] + lane; kb < k1; kb += 4 * kGradTeam) {
+                            uint4 ndb[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                const int k = kb + u * kGradTeam;
+                                ndb[u] = k < k1 ? nodes[k] : make_uint4(0u, 0u, 0u, 0u);
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                if (kb + u * kGradTeam >= k1) break;
+                                const uint4 nd = ndb[u];
+                                const unsigned fl = nd.w;
+                                const bool bra = (fl & 3u) == 1u;
+                                const int dir = (fl >> 2) & 3;
+                                const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
+                                const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
+                                const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
+                                const R c3 = n1 * (bra ? oo2z : oo2e);
+                                const R c4 = bra ? roz : roe;
+                                const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
+                                            c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
+                                V[nd.x & 0xffffu] = v;
+                            }
                         }
                     }
                     __syncthreads();
-                    for (int k = lane; k < nS; k += kGradTeam) {
-                        const R wk = k < n0 ? R(1) : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
-                        S[k] += wk * V[tgt[k]];
+                    for (int kb = lane; kb < nS; kb += 4 * kGradTeam) {
+                        int tg[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int k = kb + u * kGradTeam;
+                            tg[u] = k < nS ? tgt[k] : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int k = kb + u * kGradTeam;
+                            if (k >= nS) break;
+                            const R wk = k < n0 ? R(1) : (k < n0 + nA ? wA : (k < n0 + nA + nB ? wB : wC));
+                            S[k] += wk * V[tg[u]];
+                        }
                     }
                 }
             }
             __syncthreads();
 
-            // powers of A-B and C-D; density factors
-            if (lane < 24) {
-                const int ax = (lane % 12) >> 2, k = lane & 3;
-                const int u = lane < 12 ? atA : atC, v = lane < 12 ? atB : atD;
-                const R d = R(a.xyz[3 * u + ax] - a.xyz[3 * v + ax]);
-                R r = R(1);
-                for (int e = 0; e < k; ++e) r *= d;
-                pw[lane] = r;
+            // products of powers of A-B and C-D by power code; density factors
+            for (int c = lane; c < 128; c += kGradTeam) {
+                const int u = c < 64 ? atA : atC, v = c < 64 ? atB : atD;
+                R px[3];
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const R d = R(a.xyz[3 * u + ax] - a.xyz[3 * v + ax]);
+                    const int e = ((c & 63) >> (2 * ax)) & 3;
+                    R r = R(1);
+                    for (int k = 0; k < e; ++k) r *= d;
+                    px[ax] = r;
+                }
+                pwc[c] = px[0] * px[1] * px[2];
             }
             {
                 const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
@@ -215,7 +246,14 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
             for (int k = 0; k < 9; ++k) g[k] = R(0);
             for (int st = 0; st < nStages; ++st) {
                 const int* rec = tab + kGhSize + st * kGradStageFields;
-                const int dir = rec[0], src = rec[1], nR1 = rec[2], nCol = rec[3], nR2 = rec[4];
+                const int src = rec[1], nR1 = rec[2], nCol = rec[3], nR2 = rec[4];
+                // direction; the terms a bra / ket table row has at most (the
+                // remaining slots of a row are padding and are not read)
+                const int dir = rec[0] & 0xff, nbt = (rec[0] >> 8) & 0xff, nkt = (rec[0] >> 16) & 0xff;
+                // idx / nCol and idx / nR2 by a float reciprocal: exact for
+                // idx < 2^12 and divisors < 64 (the rounding error stays far
+                // below the 0.5 / divisor margin of idx + 0.5)
+                const float rcol = 1.0f / float(nCol), rr2 = 1.0f / float(nR2);
                 const int* bt = tab + rec[5];
                 const int* kt = tab + rec[6];
                 const int* con = tab + rec[7];
@@ -224,33 +262,33 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                 R* o2 = V + nR1 * nCol;
                 __syncthreads();                   // pw, Kv, previous stage's readers
                 for (int idx = lane; idx < nR1 * nCol; idx += kGradTeam) {
-                    const int r1 = idx / nCol, col = idx - r1 * nCol;
+                    const int r1 = __float2int_rz((float(idx) + 0.5f) * rcol), col = idx - r1 * nCol;
                     const int4 t0 = reinterpret_cast<const int4*>(bt)[2 * r1];
-                    const int4 t1 = reinterpret_cast<const int4*>(bt)[2 * r1 + 1];
+                    const int4 t1 = nbt > 4 ? reinterpret_cast<const int4*>(bt)[2 * r1 + 1] : make_int4(0, 0, 0, 0);
                     const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
                     R v = R(0);
 #pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
+                        if (u >= nbt) break;
                         const int e = tt[u];
                         const int row = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
-                        v += R(cf) * pw[pc & 3] * pw[4 + ((pc >> 2) & 3)] * pw[8 + ((pc >> 4) & 3)] *
-                             Ss[row * nCol + col];
+                        v += R(cf) * pwc[pc] * Ss[row * nCol + col];
                     }
                     o1[idx] = v;
                 }
                 __syncthreads();
                 for (int idx = lane; idx < nR1 * nR2; idx += kGradTeam) {
-                    const int r1 = idx / nR2, r2 = idx - r1 * nR2;
+                    const int r1 = __float2int_rz((float(idx) + 0.5f) * rr2), r2 = idx - r1 * nR2;
                     const int4 t0 = reinterpret_cast<const int4*>(kt)[2 * r2];
-                    const int4 t1 = reinterpret_cast<const int4*>(kt)[2 * r2 + 1];
+                    const int4 t1 = nkt > 4 ? reinterpret_cast<const int4*>(kt)[2 * r2 + 1] : make_int4(0, 0, 0, 0);
                     const int tt[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
                     R v = R(0);
 #pragma unroll
                     for (int u = 0; u < kGradHrrTerms; ++u) {
+                        if (u >= nkt) break;
                         const int e = tt[u];
                         const int colk = e & 0xfff, pc = (e >> 12) & 63, cf = (e >> 18) & 15;
-                        v += R(cf) * pw[12 + (pc & 3)] * pw[16 + ((pc >> 2) & 3)] * pw[20 + ((pc >> 4) & 3)] *
-                             o1[r1 * nCol + colk];
+                        v += R(cf) * pwc[64 + pc] * o1[r1 * nCol + colk];
                     }
                     o2[idx] = v;
                 }
@@ -313,7 +351,7 @@ static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1500 ? 256 : 6
 size_t grad_lds_bytes(const int* hdr, bool fp32) {
     const int RV = hdr[kGhSlots] > hdr[kGhMaxStage] ? hdr[kGhSlots] : hdr[kGhMaxStage];
     const int nS = hdr[kGhN0] + hdr[kGhNA] + hdr[kGhNB] + hdr[kGhNC];
-    return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 24 + 16);
+    return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 128 + 16);
 }
 
 template <int TEAM, typename R>

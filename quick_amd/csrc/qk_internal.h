@@ -95,9 +95,10 @@ constexpr float kBoysTabTmax = 36.0f;
 
 // Nuclear-gradient build (four-centre part).  The per-class tables are one
 // int32 blob (generated, grad_tables.inc) with this header, then per stage
-// kGradStageFields ints: dir (0 A, 1 B, 2 C), source set (0 S0, 1 SA, 2 SB,
-// 3 SC), bra rows, source columns, ket rows, and the offsets of the bra HRR,
-// ket HRR and contraction tables.
+// kGradStageFields ints: dir (0 A, 1 B, 2 C; bits 0-7) with the most terms of
+// any bra-table row (bits 8-15) and ket-table row (bits 16-23), source set
+// (0 S0, 1 SA, 2 SB, 3 SC), bra rows, source columns, ket rows, and the
+// offsets of the bra HRR, ket HRR and contraction tables.
 enum GradHdr {
     kGhNodes = 0, kGhLevels, kGhSlots, kGhLT, kGhN0, kGhNA, kGhNB, kGhNC, kGhF0, kGhFA, kGhFB, kGhFC,
     kGhStages, kGhNOut, kGhCA, kGhCB, kGhCC, kGhCD, kGhOffNodes, kGhOffLev, kGhOffTgt, kGhMaxStage,

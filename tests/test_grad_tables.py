@@ -88,12 +88,15 @@ def run_blob(cls, qs, Mb, K):
     setoff = [0, n0, n0 + nA, n0 + nA + nB]
     g = np.zeros(9)
     for st in range(nstages):
-        dirn, src, nR1, nCol, nR2, ob, ok, oc = b[H + SF * st: H + SF * (st + 1)]
+        dfield, src, nR1, nCol, nR2, ob, ok, oc = b[H + SF * st: H + SF * (st + 1)]
+        # direction (bits 0-7), most bra / ket terms of a row (bits 8-15 / 16-23):
+        # only those leading slots of each row are read, as on the device
+        dirn, nb, nk = dfield & 0xff, (dfield >> 8) & 0xff, (dfield >> 16) & 0xff
         Ss = S[setoff[src]:]
         # packed entries: term = row | pow << 12 | coef << 18; contraction = index | factor << 16
         unpack = lambda e: (int(e) & 0xfff, (int(e) >> 12) & 63, (int(e) >> 18) & 15)
-        bt = b[ob:ob + nR1 * NT].reshape(nR1, NT)
-        kt = b[ok:ok + nR2 * NT].reshape(nR2, NT)
+        bt = b[ob:ob + nR1 * NT].reshape(nR1, NT)[:, :nb]
+        kt = b[ok:ok + nR2 * NT].reshape(nR2, NT)[:, :nk]
         con = b[oc:oc + nout * 4].reshape(nout, 4)
         o1 = np.zeros(nR1 * nCol)
         for idx in range(nR1 * nCol):
