@@ -142,6 +142,16 @@ int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
 int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 
+/* The forward/backward pair under the names the test harness binds (the
+ * torch.autograd.Function there wraps them); host arrays, synchronous.
+ *   qk_fock_build: F <- F + G(D), as qk_fock_host (gpu_get2e_, gpu.cu:1806).
+ *   qk_fock_grad:  grad <- grad + dE2/dR, E2 = 1/2 sum D G(D), as qk_grad_host
+ *                  (gpu_grad_, gpu.cu:1481).  W, the energy-weighted density,
+ *                  belongs to the one-electron overlap term (gradient.f90:85-237),
+ *                  which this library does not compute: it must be NULL. */
+int qk_fock_build(qk_ctx* c, const double* D, double* F, const qk_opts* o);
+int qk_fock_grad(qk_ctx* c, const double* D, const double* W, double* grad, const qk_opts* o);
+
 /* Statistics of the last build: surviving contracted Qshell quartets, primitive
  * quartets evaluated (FP64 and FP32), algorithmic flops. */
 typedef struct qk_stats {

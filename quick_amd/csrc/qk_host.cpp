@@ -1151,6 +1151,20 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
     return 0;
 }
 
+// the forward/backward pair of the test harness (same operations as the
+// host entry points above)
+extern "C" int qk_fock_build(qk_ctx* c, const double* D, double* F, const qk_opts* o) {
+    return qk_fock_host(c, D, F, o);
+}
+
+extern "C" int qk_fock_grad(qk_ctx* c, const double* D, const double* W, double* grad, const qk_opts* o) {
+    if (W) {
+        set_err("qk_fock_grad: W (energy-weighted density) must be NULL: the one-electron terms are not computed here");
+        return -4;
+    }
+    return qk_grad_host(c, D, grad, o);
+}
+
 // surviving quartets and primitive quartets of class k in the last gradient
 // (FP64 part), and its kernel time
 extern "C" int qk_grad_class_stats(qk_ctx* c, int k, long long* nq, long long* np, float* ms) {

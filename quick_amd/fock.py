@@ -79,6 +79,9 @@ def lib() -> C.CDLL:
                                      C.POINTER(C.c_double), C.POINTER(C.c_float)]
         L.qk_grad_class_stats.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                           C.POINTER(C.c_float)]
+        L.qk_fock_build.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(QkOpts)]
+        L.qk_fock_grad.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(QkOpts)]
         L.qk_grad_class_stats_fp32.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                                C.POINTER(C.c_float)]
         L.qk_class_stats_fp32.argtypes = [P, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
