@@ -50,13 +50,16 @@ QK_GRAD_LIGHT(2, 1, 1, 0, dpps)
 QK_GRAD_LIGHT(2, 2, 0, 0, ddss)
 #undef QK_GRAD_LIGHT
 
-// Two wavefronts per SIMD when one pass's accumulators and density factors fit
-// the budget of the generator (no spills), else one; FP32 halves the registers.
+// Two wavefronts per SIMD (256 VGPRs a lane) only while one pass's
+// accumulators and density factors leave the VRR room there: up to 104 values
+// in FP32, 64 in FP64; otherwise one wavefront per SIMD, with 512 registers,
+// spills far less and runs faster (measured per class on (H2O)32: ppps FP64
+// 19.8 -> 8.6 ms, dspp FP32 4.3 -> 2.0 ms; dsps FP32 and dsss FP64 prefer two).
 template <int LA, int LB, int LC, int LD, typename R>
 struct GradLightOcc {
     using S = typename GradLight<LA, LB, LC, LD>::Sets;
     static constexpr int nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-    static constexpr int value = (S::kWidth + nout) * (int)sizeof(R) <= 104 * 8 ? 2 : 1;
+    static constexpr int value = S::kWidth + nout <= (sizeof(R) == 8 ? 64 : 104) ? 2 : 1;
 };
 
 // primitive pairs of the bra for the gradient: the Fock slab's seven fields
