@@ -30,8 +30,8 @@ struct TeamLayout {
     static constexpr int LT = LA + LB + LC + LD;
     static constexpr int kV = TT::kSlots > NAB * TT::kNF + NOUT ? TT::kSlots : NAB * TT::kNF + NOUT;
     static constexpr int kDbl = NCD + NAB + NB * ND + NB * NC + NA * ND + NA * NC;
-    // V | acc | D blocks | powers of AB, CD (3 axes x 3) | F_m
-    static constexpr int oAcc = kV, oD = oAcc + NACC, oPw = oD + kDbl, oF = oPw + 18;
+    // V | acc | D blocks | products of AB, CD powers by power code (64 + 64) | F_m
+    static constexpr int oAcc = kV, oD = oAcc + NACC, oPw = oD + kDbl, oF = oPw + 128;
     static constexpr int total = oF + 16;
 };
 
@@ -85,7 +85,9 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
     R* Dbc = Dbd + NB * ND;
     R* Dad = Dbc + NB * NC;
     R* Dac = Dad + NA * ND;
-    R* pw = lds + Lo::oPw;        // pw[axis*3 + k] = AB_axis^k, pw[9 + axis*3 + k] = CD_axis^k
+    // pwc[c] = AB_x^(c&3) AB_y^((c>>2)&3) AB_z^((c>>4)&3) for the power code c of a
+    // bra HRR term; pwc[64 + c] the same in C-D for the ket HRR
+    R* pwc = lds + Lo::oPw;
     R* Fl = lds + Lo::oF;
     __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
     for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
@@ -161,8 +163,20 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     }
                     for (int L = 1; L < TT::kLevels; ++L) {
                         __syncthreads();
-                        for (int k = lev[L] + lane; k < lev[L + 1]; k += 64) {
-                            const uint4 nd = nodes[k];
+                        const int k1 = lev[L + 1];
+                        // the descriptors of up to four of the lane's nodes are
+                        // loaded before any is evaluated (one memory latency)
+                        for (int kb = lev[L] + lane; kb < k1; kb += 4 * 64) {
+                          uint4 ndb[4];
+#pragma unroll
+                          for (int u = 0; u < 4; ++u) {
+                              const int k = kb + u * 64;
+                              ndb[u] = k < k1 ? nodes[k] : make_uint4(0u, 0u, 0u, 0u);
+                          }
+#pragma unroll
+                          for (int u = 0; u < 4; ++u) {
+                            if (kb + u * 64 >= k1) break;
+                            const uint4 nd = ndb[u];
                             const unsigned fl = nd.w;
                             const bool bra = (fl & 3u) == 1u;
                             const int dir = (fl >> 2) & 3;
@@ -174,6 +188,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                             const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
                                         c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
                             V[nd.x & 0xffffu] = v;
+                          }
                         }
                     }
                     __syncthreads();
@@ -182,14 +197,19 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
             }
             __syncthreads();
 
-            // HRR: powers of A-B and C-D, density blocks
-            if (lane < 18) {
-                const R Bx = R(a.xyz[3 * atB]), By = R(a.xyz[3 * atB + 1]), Bz = R(a.xyz[3 * atB + 2]);
-                const R Dx = R(a.xyz[3 * atD]), Dy = R(a.xyz[3 * atD + 1]), Dz = R(a.xyz[3 * atD + 2]);
-                const int grp = lane / 9, axis = (lane % 9) / 3, k = lane % 3;
-                const R d = grp == 0 ? (axis == 0 ? Ax - Bx : (axis == 1 ? Ay - By : Az - Bz))
-                                     : (axis == 0 ? Cx - Dx : (axis == 1 ? Cy - Dy : Cz - Dz));
-                pw[lane] = k == 0 ? R(1) : (k == 1 ? d : d * d);
+            // HRR: products of powers of A-B and C-D by power code, density blocks
+            for (int c = lane; c < 128; c += 64) {
+                const int u = c < 64 ? atA : atC, v = c < 64 ? atB : atD;
+                R px[3];
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const R d = R(a.xyz[3 * u + ax]) - R(a.xyz[3 * v + ax]);
+                    const int e = ((c & 63) >> (2 * ax)) & 3;
+                    R r = R(1);
+                    for (int k = 0; k < e; ++k) r *= d;
+                    px[ax] = r;
+                }
+                pwc[c] = px[0] * px[1] * px[2];
             }
             const int fa = a.q.f0[qa], fb = a.q.f0[qb], fc = a.q.f0[qc], fd = a.q.f0[qd];
             const auto* Dp = in.Dp;
@@ -210,8 +230,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int e = tt[u] & 0xfff, pw3 = (tt[u] >> 12) & 63, cf = (tt[u] >> 18) & 15;
-                    const R c = R(cf) * pw[pw3 & 3] * pw[3 + ((pw3 >> 2) & 3)] * pw[6 + ((pw3 >> 4) & 3)];
-                    v += c * acc[e * NF + f];
+                    v += R(cf) * pwc[pw3] * acc[e * NF + f];
                 }
                 hb[h] = v;
             }
@@ -225,8 +244,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int f = tt[u] & 0xfff, pw3 = (tt[u] >> 12) & 63, cf = (tt[u] >> 18) & 15;
-                    const R c = R(cf) * pw[9 + (pw3 & 3)] * pw[12 + ((pw3 >> 2) & 3)] * pw[15 + ((pw3 >> 4) & 3)];
-                    v += c * hb[ab * NF + f];
+                    v += R(cf) * pwc[64 + pw3] * hb[ab * NF + f];
                 }
                 outv[o] = v;
             }
