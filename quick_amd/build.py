@@ -23,6 +23,10 @@ HIP_FLAGS = [
     # no packed-FP32 instructions (v_pk_*_f32) in any generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # FP32 division and square root by the hardware approximations (the FP32
+    # kernels serve the mixed-precision path, whose error budget is 1e-6;
+    # FP64 code is unaffected)
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-unused-result",
 ]
 

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     const R zcd = PP.zeta[kj], oo2e = PP.oo2z[kj];
                     const R wC = R(2) * ea[kj];
                     const R Qx = PP.Px[kj], Qy = PP.Py[kj], Qz = PP.Pz[kj];
-                    const R inv = R(1) / (zab + zcd);
+                    const R inv = recip(zab + zcd);
                     const R roz = zcd * inv, roe = zab * inv;
                     const R ROU = zab * roz;
                     const R Wx = (Px * zab + Qx * zcd) * inv;

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
                         if (M == Ops::Sets::kMask[0]) ++nprimq;
                         const R zab = bra[i], oo2z = bra[64 + i];
                         const R Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                        const R inv = R(1) / (zab + zcd);
+                        const R inv = recip(zab + zcd);
                         const R roz = zcd * inv, roe = zab * inv;
                         const R ROU = zab * roz;
                         const R Wx = (Px * zab + Qx * zcd) * inv;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     ++nprimq;
                     const R zcd = in.zeta[kj], oo2e = in.oo2z[kj];
                     const R Qx = in.Px[kj], Qy = in.Py[kj], Qz = in.Pz[kj];
-                    const R inv = R(1) / (zab + zcd);
+                    const R inv = recip(zab + zcd);
                     const R roz = zcd * inv, roe = zab * inv;
                     const R ROU = zab * roz;
                     const R Wx = (Px * zab + Qx * zcd) * inv;
