@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     const R zcd = PP.zeta[kj], oo2e = PP.oo2z[kj];
                     const R wC = R(2) * ea[kj];
                     const R Qx = PP.Px[kj], Qy = PP.Py[kj], Qz = PP.Pz[kj];
-                    const R inv = recip(zab + zcd);
+                    const R rs = rsqrt_nr(zab + zcd);
+                    const R inv = rs * rs;
                     const R roz = zcd * inv, roe = zab * inv;
                     const R ROU = zab * roz;
                     const R Wx = (Px * zab + Qx * zcd) * inv;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     R F[kMaxF + 1];
                     if constexpr (f64) boys_ref_n(T, Mb, LT2, F);
                     else boys_f32_all(T, a.boys_tab, F);
-                    const R s = X2 * PP.K[kj] * sqrt_r(inv);
+                    const R s = X2 * PP.K[kj] * rs;
                     R fsel = F[0] * s;
 #pragma unroll
                     for (int m = 1; m <= kMaxF; ++m) fsel = (lane == m) ? F[m] * s : fsel;

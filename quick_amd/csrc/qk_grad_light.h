@@ -207,7 +207,8 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
                         if (M == Ops::Sets::kMask[0]) ++nprimq;
                         const R zab = bra[i], oo2z = bra[64 + i];
                         const R Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                        const R inv = recip(zab + zcd);
+                        const R rs = rsqrt_nr(zab + zcd);
+                        const R inv = rs * rs;
                         const R roz = zcd * inv, roe = zab * inv;
                         const R ROU = zab * roz;
                         const R Wx = (Px * zab + Qx * zcd) * inv;
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
                         R F[LT2 + 1];
                         if constexpr (f64) boys_fit<LT2>(T, Mb, s_boys, F);
                         else boys_f32<LT2>(T, a.boys_tab, F);
-                        const R s = X0 * bra[5 * 64 + i] * Kcd * sqrt_r(inv);
+                        const R s = X0 * bra[5 * 64 + i] * Kcd * rs;
 #pragma unroll
                         for (int m = 0; m <= LT2; ++m) F[m] *= s;
                         Ops::template vrg<M, R>(F, Px - Ax, Py - Ay, Pz - Az, Wx - Px, Wy - Py, Wz - Pz, QCx,

@@ -108,20 +108,30 @@ __device__ __forceinline__ double recip(double x) {
 }
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// 1/sqrt(x) for the same sums: the hardware estimate refined by two Newton
+// steps, y <- y + y (1 - x y^2) / 2 (FP64); the hardware estimate (FP32).  A
+// primitive quartet takes 1/(zeta+eta) as its square (one rounding more).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y * 0.5, fma(-x * y, y, 1.0), y);
+    return fma(y * 0.5, fma(-x * y, y, 1.0), y);
+}
+__device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+
 template <int LT>
 __device__ __forceinline__ void boys_fit(double X, int Mb, const double* __restrict__ tab, double* F) {
-    const double PIE4 = 3.1415926535897932384626433832795 / 4.0;
     const double E = exp(-X);
     const bool up = (X > 3.0e-5) || (X > 3.0e-7 && Mb < 4);
     if (up) {
-        const double XINV = recip(X);
+        const double RS = rsqrt_nr(X);           // 1/sqrt(X); 1/X as its square
+        const double XINV = RS * RS;
         const int r = int(X > 1.0) + int(X > 3.0) + int(X > 5.0) + int(X > 10.0) + int(X > 15.0) + int(X > 33.0);
         const double z = r == 0 ? X : (r == 1 ? X - 2.0 : (r == 2 ? X - 4.0 : XINV));
         const double* c = tab + kBoysFitTerms * r;
         double v = c[0];
 #pragma unroll
         for (int k = 1; k < kBoysFitTerms; ++k) v = fma(v, z, c[k]);
-        const double s = sqrt(PIE4 * XINV);
+        const double s = 0.88622692545275801365 * RS;   // sqrt(PIE4 / X), sqrt(pi/4) = 0.886...
         F[0] = r <= 2 ? fma(v, X + X, E) : fma(v, E, s);
 #pragma unroll
         for (int m = 1; m <= LT; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
@@ -340,7 +350,8 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                         ++nprimq;
                         const double zab = bra[i], oo2z = bra[64 + i];
                         const double Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                        const double inv = recip(zab + zcd);   // the one reciprocal per primitive quartet
+                        const double rs = rsqrt_nr(zab + zcd);   // 1/sqrt(zeta+eta): reciprocal and root in one
+                        const double inv = rs * rs;
                         const double roz = zcd * inv, roe = zab * inv;
                         const double ROU = zab * roz;
                         const double Wx = (Px * zab + Qx * zcd) * inv;
@@ -350,7 +361,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                         const double T = (dx * dx + dy * dy + dz * dz) * ROU;
                         double F[LT + 1];
                         boys_fit<LT>(T, Mb, s_boys, F);
-                        const double s = X0 * bra[5 * 64 + i] * Kcd * sqrt(inv);
+                        const double s = X0 * bra[5 * 64 + i] * Kcd * rs;
 #pragma unroll
                         for (int m = 0; m <= LT; ++m) F[m] *= s;
                         ClassOps<LA, LB, LC, LD>::template vrr<double>(
@@ -614,7 +625,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         ++nprimq;
                         const float zab = bra[i], oo2z = bra[64 + i];
                         const float Px = bra[2 * 64 + i], Py = bra[3 * 64 + i], Pz = bra[4 * 64 + i];
-                        const float inv = recip(zab + zcd);
+                        const float rs = rsqrt_nr(zab + zcd);
+                        const float inv = rs * rs;
                         const float roz = zcd * inv, roe = zab * inv;
                         const float ROU = zab * roz;
                         const float Wx = (Px * zab + Qx * zcd) * inv;
@@ -624,7 +636,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         const float T = (dx * dx + dy * dy + dz * dz) * ROU;
                         float F[LT + 1];
                         boys_f32<LT>(T, a.boys_tab, F);
-                        const float s = X0 * bra[5 * 64 + i] * Kcd * sqrtf(inv);
+                        const float s = X0 * bra[5 * 64 + i] * Kcd * rs;
 #pragma unroll
                         for (int m = 0; m <= LT; ++m) F[m] *= s;
                         ClassOps<LA, LB, LC, LD>::template vrr<float>(

@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     ++nprimq;
                     const R zcd = in.zeta[kj], oo2e = in.oo2z[kj];
                     const R Qx = in.Px[kj], Qy = in.Py[kj], Qz = in.Pz[kj];
-                    const R inv = recip(zab + zcd);
+                    const R rs = rsqrt_nr(zab + zcd);
+                    const R inv = rs * rs;
                     const R roz = zcd * inv, roe = zab * inv;
                     const R ROU = zab * roz;
                     const R Wx = (Px * zab + Qx * zcd) * inv;
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     const R T = (dx * dx + dy * dy + dz * dz) * ROU;
                     R F[LT + 1];
                     in.template boys<LT>(T, Mb, s_boys, a, F);
-                    const R s = X2 * in.K[kj] * sqrt(inv);
+                    const R s = X2 * in.K[kj] * rs;
                     R fsel = F[0] * s;
 #pragma unroll
                     for (int m = 1; m <= LT; ++m) fsel = (lane == m) ? F[m] * s : fsel;
