@@ -158,22 +158,26 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
         if (qa == qb) wq *= R(0.5);
         if (qc == qd) wq *= R(0.5);
         if (p == q) wq *= R(0.5);
-        R K[NOUT];
+        // formed (from the six density blocks, re-read from memory) just before
+        // each pass's contraction, so that neither K nor D is live in the
+        // register-heavy primitive loop
+        auto form_K = [&](R* K) {
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+            for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
-            for (int ib = 0; ib < NB; ++ib)
+                for (int ib = 0; ib < NB; ++ib)
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
+                    for (int ic = 0; ic < NC; ++ic)
 #pragma unroll
-                    for (int id = 0; id < ND; ++id) {
-                        const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
-                        const R Dab = Dp[ra + fb + ib], Dcd = Dp[(long long)(fc + ic) * n + fd + id];
-                        const R Dac = Dp[ra + fc + ic], Dbd = Dp[rb + fd + id];
-                        const R Dad = Dp[ra + fd + id], Dbc = Dp[rb + fc + ic];
-                        K[((ia * NB + ib) * NC + ic) * ND + id] =
-                            wq * (R(4) * Dab * Dcd - hyb * (Dac * Dbd + Dad * Dbc));
-                    }
+                        for (int id = 0; id < ND; ++id) {
+                            const long long ra = (long long)(fa + ia) * n, rb = (long long)(fb + ib) * n;
+                            const R Dab = Dp[ra + fb + ib], Dcd = Dp[(long long)(fc + ic) * n + fd + id];
+                            const R Dac = Dp[ra + fc + ic], Dbd = Dp[rb + fd + id];
+                            const R Dad = Dp[ra + fd + id], Dbc = Dp[rb + fc + ic];
+                            K[((ia * NB + ib) * NC + ic) * ND + id] =
+                                wq * (R(4) * Dab * Dcd - hyb * (Dac * Dbd + Dad * Dbc));
+                        }
+        };
         R g[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[k] = R(0);
@@ -229,6 +233,8 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
                     }
                 }
             }
+            R K[NOUT];
+            form_K(K);
             Ops::template gcon<M, R>(acc0, accA, accB, accC, Ax - Bx, Ay - By, Az - Bz, Cx - Dx, Cy - Dy,
                                      Cz - Dz, K, g);
         };
