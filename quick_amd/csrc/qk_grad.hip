@@ -344,7 +344,9 @@ __global__ void grad_finalize(const unsigned long long* __restrict__ G, double* 
 }
 
 // four wavefronts per quartet for the classes with the largest node programs
-static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1500 ? 256 : 64; }
+// (dpdp, dddp, dddd); measured on (H2O)32, ddds and ddpp run as fast or faster
+// with one (ddds FP64 9.4 -> 7.5 ms) and dppp-sized classes twice as fast
+static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1800 ? 256 : 64; }
 
 // dynamic LDS of grad_kernel: the FP64 partial sums (9 x 4), then the VRR /
 // stage buffers, sets, density factors, powers and Boys values in the
