@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
     extern __shared__ unsigned long long s_grad[];   // 3 x natom when a.lds_grad
-    __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
+    __shared__ double s_boys[f64 ? kBoysFitRegions * kBoysFitTerms : 1];
+    __shared__ float s_btab[f64 ? 1 : kBoysTabRows * kBoysLdsCols];
     __shared__ R s_bra[4][9 * 64];
     R* bra = s_bra[threadIdx.x >> 6];
     const bool use_lds = a.lds_grad != 0;
@@ -125,6 +126,8 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
     const R rcut = R(a.cutoff), hyb = R(a.hybrid);
     if constexpr (f64)
         for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    else
+        stage_boys_f32<LT2>(a.boys_tab, s_btab);
     if (use_lds)
         for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x) s_grad[k] = 0ull;
     __syncthreads();
@@ -222,7 +225,7 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
                         const R T = (dx * dx + dy * dy + dz * dz) * ROU;
                         R F[LT2 + 1];
                         if constexpr (f64) boys_fit<LT2>(T, Mb, s_boys, F);
-                        else boys_f32<LT2>(T, a.boys_tab, F);
+                        else boys_f32<LT2>(T, s_btab, F);
                         const R s = X0 * bra[5 * 64 + i] * Kcd * rs;
 #pragma unroll
                         for (int m = 0; m <= LT2; ++m) F[m] *= s;

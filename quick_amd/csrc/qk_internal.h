@@ -88,10 +88,13 @@ struct DevPrimsF {
 };
 
 // Boys table for the FP32 path: F_m(T_k), T_k = k / kBoysTabPerUnit, m < kBoysTabM
-constexpr int kBoysTabPerUnit = 16;
+constexpr int kBoysTabPerUnit = 8;
 constexpr int kBoysTabRows = 36 * kBoysTabPerUnit + 1;
 constexpr int kBoysTabM = 16;
 constexpr float kBoysTabTmax = 36.0f;
+// the kernels keep the kBoysLdsCols orders they expand (F_LT .. F_LT+4 of every
+// grid point) in LDS: kBoysTabRows x kBoysLdsCols floats (5.8 KB)
+constexpr int kBoysLdsCols = 5;
 
 // Nuclear-gradient build (four-centre part).  The per-class tables are one
 // int32 blob (generated, grad_tables.inc) with this header, then per stage

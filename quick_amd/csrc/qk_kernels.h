@@ -475,17 +475,36 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
 // FP32 path (mixed precision): quartets whose Schwarz bound Y_ij*Y_kl lies
 // below the FP32 threshold.  QUICK's Boys fit recurs upward, which is unusable
 // in single precision at small T, so this path uses a tabulated Taylor
-// expansion of F_LT about the nearest grid point (5 terms, grid 1/16) and the
+// expansion of F_LT about the nearest grid point (5 terms, grid 1/8: the
+// truncation |d|^5/5! <= 8e-9 relative is below FP32 resolution) and the
 // stable downward recursion; beyond T = 36 the asymptotic form and upward
 // recursion (the exp(-T) terms are below FP32 resolution there).  The operator
 // contributions are promoted to FP64 before the fixed-point accumulation.
+//
+// The light kernels of the small classes stage the five orders they expand in
+// LDS once per block (the per-primitive gather then costs an LDS round trip
+// instead of a memory one); where those 5.8 KB would cost resident blocks
+// (larger exchange rows, the one-wavefront team kernels) the table is read from
+// memory (measured on (H2O)32: (ss|ss) FP32 1.85 -> 1.36 ms with it, (ds|ps)
+// 1.28 -> 1.57 ms, dppp gradient 13.5 -> 21.9 ms).
 template <int LT>
-__device__ __forceinline__ void boys_f32(float T, const float* __restrict__ tab, float* F) {
+__device__ __forceinline__ void stage_boys_f32(const float* __restrict__ tab, float* s_tab) {
+    for (int i = threadIdx.x; i < kBoysTabRows * kBoysLdsCols; i += blockDim.x) {
+        const int k = i / kBoysLdsCols, j = i - k * kBoysLdsCols;
+        s_tab[i] = tab[k * kBoysTabM + LT + j];
+    }
+}
+
+// F_0..F_LT(T) from orders LT..LT+4 of the table: tab points at order LT of
+// grid point 0, rows kStride apart (the LDS slice: kBoysLdsCols; the table in
+// memory: a.boys_tab + LT and kBoysTabM)
+template <int LT, int kStride = kBoysLdsCols>
+__device__ __forceinline__ void boys_f32(float T, const float* tab, float* F) {
     if (T < kBoysTabTmax) {
         const int k = __float2int_rn(T * float(kBoysTabPerUnit));
         const float d = float(k) * (1.0f / float(kBoysTabPerUnit)) - T;
-        const float* r = tab + k * kBoysTabM + LT;
-        // F_LT(T) = sum_j F_{LT+j}(T_k) d^j / j!,  d = T_k - T, |d| <= 1/32
+        const float* r = tab + k * kStride;
+        // F_LT(T) = sum_j F_{LT+j}(T_k) d^j / j!,  d = T_k - T, |d| <= 1/16
         F[LT] = (((r[4] * (d * 0.25f) + r[3]) * (d * (1.0f / 3.0f)) + r[2]) * (d * 0.5f) + r[1]) * d + r[0];
         const float e = __expf(-T);
         const float t2 = T + T;
@@ -520,12 +539,13 @@ __device__ __forceinline__ double sqrt_r(double x) { return sqrt(x); }
 __device__ __forceinline__ float sqrt_r(float x) { return sqrtf(x); }
 
 // F_0..F_kMaxGradF(T) in FP32 for the gradient's table-driven kernel: the
-// expansion of boys_f32 at the highest order the table supports for it, then
-// downward recursion (stable), so every order is valid whatever the class
+// expansion of boys_f32 at the highest order the table supports for it (read
+// from the table in memory), then downward recursion (stable), so every order
+// is valid whatever the class
 constexpr int kMaxGradF = 10;
 static_assert(kMaxGradF + 4 < kBoysTabM, "Boys table too narrow for the gradient");
-__device__ __forceinline__ void boys_f32_all(float T, const float* __restrict__ tab, float* F) {
-    boys_f32<kMaxGradF>(T, tab, F);
+__device__ __forceinline__ void boys_f32_all(float T, const float* tab, float* F) {
+    boys_f32<kMaxGradF, kBoysTabM>(T, tab + kMaxGradF, F);
 }
 
 __device__ __forceinline__ void fx_add_f(unsigned long long* X, long long idx, float v) {
@@ -556,6 +576,12 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const float plim = float(a.prim_limit);
     __shared__ float s_braf[4][7 * 64];
     float* bra = s_braf[threadIdx.x >> 6];
+    constexpr bool kBoysLds = NA + NB <= 4;   // two or four exchange rows
+    __shared__ float s_btab[kBoysLds ? kBoysTabRows * kBoysLdsCols : 1];
+    if constexpr (kBoysLds) {
+        stage_boys_f32<LT>(a.boys_tab, s_btab);
+        __syncthreads();
+    }
 
     // exchange rows of the current bra in LDS (run mode); dynamic shared memory
     extern __shared__ unsigned long long s_xrows[];
@@ -635,7 +661,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         const float dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
                         const float T = (dx * dx + dy * dy + dz * dz) * ROU;
                         float F[LT + 1];
-                        boys_f32<LT>(T, a.boys_tab, F);
+                        if constexpr (kBoysLds) boys_f32<LT>(T, s_btab, F);
+                        else boys_f32<LT, kBoysTabM>(T, a.boys_tab + LT, F);
                         const float s = X0 * bra[5 * 64 + i] * Kcd * rs;
 #pragma unroll
                         for (int m = 0; m <= LT; ++m) F[m] *= s;

@@ -51,7 +51,7 @@ struct TeamIn<double> {
     __device__ explicit TeamIn(const FockArgs& a)
         : zeta(a.pp.zeta), oo2z(a.pp.oo2z), Px(a.pp.Px), Py(a.pp.Py), Pz(a.pp.Pz), K(a.pp.K), cut(a.pp.cut), Dp(a.Dp) {}
     template <int LT>
-    __device__ __forceinline__ void boys(double T, int Mb, const double* fit, const FockArgs&, double* F) const {
+    __device__ __forceinline__ void boys(double T, int Mb, const double* fit, double* F) const {
         boys_fit<LT>(T, Mb, fit, F);
     }
 };
@@ -60,10 +60,11 @@ struct TeamIn<float> {
     const float *zeta, *oo2z, *Px, *Py, *Pz, *K, *cut, *Dp;
     __device__ explicit TeamIn(const FockArgs& a)
         : zeta(a.ppf.zeta), oo2z(a.ppf.oo2z), Px(a.ppf.Px), Py(a.ppf.Py), Pz(a.ppf.Pz), K(a.ppf.K), cut(a.ppf.cut),
-          Dp(a.Dpf) {}
+          Dp(a.Dpf), tab(a.boys_tab) {}
+    const float* tab;
     template <int LT>
-    __device__ __forceinline__ void boys(float T, int, const double*, const FockArgs& a, float* F) const {
-        boys_f32<LT>(T, a.boys_tab, F);
+    __device__ __forceinline__ void boys(float T, int, const double*, float* F) const {
+        boys_f32<LT, kBoysTabM>(T, tab + LT, F);
     }
 };
 
@@ -89,8 +90,11 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
     // bra HRR term; pwc[64 + c] the same in C-D for the ket HRR
     R* pwc = lds + Lo::oPw;
     R* Fl = lds + Lo::oF;
-    __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
-    for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
+    // QUICK's fit coefficients (the FP32 path reads its Boys table from memory)
+    constexpr bool f64 = sizeof(R) == 8;
+    __shared__ double s_boys[f64 ? kBoysFitRegions * kBoysFitTerms : 1];
+    if constexpr (f64)
+        for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
     __syncthreads();
     const TeamIn<R> in(a);
     const R X0 = R(34.98683665524972);
@@ -147,7 +151,7 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     const R dx = Px - Qx, dy = Py - Qy, dz = Pz - Qz;
                     const R T = (dx * dx + dy * dy + dz * dz) * ROU;
                     R F[LT + 1];
-                    in.template boys<LT>(T, Mb, s_boys, a, F);
+                    in.template boys<LT>(T, Mb, s_boys, F);
                     const R s = X2 * in.K[kj] * rs;
                     R fsel = F[0] * s;
 #pragma unroll

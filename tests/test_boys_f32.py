@@ -1,8 +1,8 @@
 """The FP32 kernels' Boys evaluation (mixed precision).
 
-The library tabulates F_m(T_k), T_k = k/16, in FP64 and rounds to FP32; the
-kernels evaluate F_LT(T) by a 5-term Taylor expansion about the nearest grid
-point, then recur downwards (F_m = (2T F_{m+1} + e^{-T}) / (2m+1)), and use the
+The library tabulates F_m(T_k), T_k = k/8, in FP64 and rounds to FP32; the
+kernels stage the five orders they expand in LDS and evaluate F_LT(T) by a
+5-term Taylor expansion about the nearest grid point, then recur downwards (F_m = (2T F_{m+1} + e^{-T}) / (2m+1)), and use the
 asymptotic form sqrt(pi/4T) with upward recursion from T = 36 on.  Both the
 table and that evaluation (emulated here in numpy float32, operation by
 operation) are checked against the incomplete-gamma closed form
@@ -93,5 +93,5 @@ def test_fp32_evaluation_error_bound(table, LT):
         for m in range(LT + 1):
             ref = boys_closed_form(m, np.array([T]))[0]
             worst = max(worst, abs(F[m] - ref) / ref)
-    # a few FP32 ulps: the Taylor truncation (|d|^5/120 ~ 2.6e-10) is far below
+    # a few FP32 ulps: the Taylor truncation (|d|^5/120 <= 8e-9, |d| <= 1/16) is below
     assert worst < 1.0e-6, worst
