@@ -145,14 +145,15 @@ def main():
     sptr = stream.cuda_stream
 
     def timed(opts, label):
-        """warmup + `steps` timed builds with `opts`; returns ms/build, G, per-kernel rows"""
+        """warmup + `steps` timed builds with `opts` (no per-kernel events inside
+        the timed region), then a few builds with per-kernel HIP events for the
+        per-class figures; returns ms/build, G, per-kernel rows, stats"""
         eng.prepare(opts)
+        eng.set_timing(False)
         for _ in range(args.warmup):
             eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
         torch.cuda.synchronize()
-        eng.set_timing(True)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev0.record(stream)
         for _ in range(args.steps):
@@ -161,13 +162,19 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         ms = ev0.elapsed_time(ev1) / args.steps
+        G = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
+        # per-class kernel times (HIP events around every class launch)
+        eng.set_timing(True)
+        for _ in range(max(1, min(args.steps, 5))):
+            eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
+        torch.cuda.synchronize()
         log(f"{label}: {args.steps} builds {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build")
         # kernel rows: (kernel, tag, quartets, prim quartets, flops, ms, precision)
         rows = [(f"{kernel_of(c[0])}<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp64") for c in eng.class_stats()]
         rows += [(f"{kernel_of(c[0])}_f32<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp32")
                  for c in eng.class_stats_fp32()]
         rows = [r for r in rows if r[2]]
-        G = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
+        eng.set_timing(False)
         return ms, G, rows, eng.stats()
 
     def show(rows):
@@ -236,7 +243,7 @@ def main():
             ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)   # warm-up (builds the work lists)
             torch.cuda.synchronize()
             gval = gout.cpu().numpy()
-            ge.set_timing(True)
+            ge.set_timing(False)
             nrep = max(1, min(args.steps, 5))
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
@@ -245,7 +252,11 @@ def main():
             e1.record(stream)
             torch.cuda.synchronize()
             gs = e0.elapsed_time(e1) / nrep / 1e3
+            ge.set_timing(True)                   # one more, with per-class events
+            ge.grad_device(gDd.data_ptr(), gout.data_ptr(), gopts, sptr)
+            torch.cuda.synchronize()
             st64, st32 = ge.grad_class_stats(), ge.grad_class_stats(fp32=True)
+            ge.set_timing(False)
             lab = gprec if gthr is None else f"mixed {gthr:.0e}"
             for sub, gst in (("", st64), (" fp32", st32)):
                 for c in sorted(gst, key=lambda c: -c[3]):

@@ -13,14 +13,47 @@
 // first ket, length <= 64} (bra and kets from the class- and bound-sorted pair
 // lists, so the kets of a tile are consecutive and all pass the Schwarz test).
 // A wavefront takes one tile at a time: the bra's pair and primitive data are
-// broadcast loads and the kets' primitive-major data are coalesced.  The
-// heavy d classes use the wavefront-cooperative kernel (qk_team.h) instead.
+// broadcast loads and the kets' primitive-major data are coalesced.  Where a
+// block takes one bra pair at a time (exchange rows in LDS), the density
+// screen runs first in compact_kernel, which packs each segment's surviving
+// kets, so that the class kernel's lanes are all busy (about 40% of the Schwarz
+// candidates fail the density test, scattered over the tiles).  The heavy d
+// classes use the wavefront-cooperative kernel (qk_team.h) instead.
 #include "qk_kernels.h"
 
 namespace qk {
 
 // ---------------------------------------------------------------------------
 // helper kernels
+
+// Density screen of the candidate quartets, compacted: one wavefront per
+// segment (a bra pair and a ket range of one group) tests Y_p Y_q DNmax > tau
+// for its candidate kets 64 at a time and appends the survivors in order
+// (ballot + prefix count) at list[segs[s].w ...]; counts[s] gets their number.
+// The class kernels then fill all 64 lanes with surviving quartets.
+__global__ void __launch_bounds__(256) compact_kernel(const FockArgs a, const int4* __restrict__ segs, long long nsegs,
+                                                      int* __restrict__ list, int* __restrict__ counts) {
+    const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (s >= nsegs) return;   // the whole wavefront
+    const int4 sg = segs[s];
+    const int qa = a.pr.qa[sg.x], qb = a.pr.qb[sg.x];
+    const int sA = a.q.shell[qa], sB = a.q.shell[qb];
+    const double yp = a.pr.Y[sg.x];
+    int cnt = 0;
+    for (int q0 = sg.y; q0 < sg.z; q0 += 64) {
+        const int q = q0 + lane;
+        bool keep = false;
+        if (q < sg.z) {
+            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, a.q.shell[a.pr.qa[q]], a.q.shell[a.pr.qb[q]]);
+            keep = a.pr.Y[q] * yp * dn > a.tau;   // the class kernels' test, same operation order
+        }
+        const unsigned long long bal = __ballot(keep);
+        if (keep) list[sg.w + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = q;
+        cnt += __popcll(bal);
+    }
+    if (lane == 0) counts[s] = cnt;
+}
 
 // Dp = diag(cons) D diag(cons): folds QUICK's per-function cons factors
 // (applied inside hrrwhole, subs/hrr.f90:261-295) into the density; the
@@ -169,6 +202,10 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
                            hipStream_t s) {
     hipLaunchKernelGGL(density_cutoff, dim3(blocks_for((long long)ns * ns, 256)), dim3(256), 0, s, D, sh_f0, sh_nf,
                        cm, n, ns);
+}
+void launch_compact(const FockArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s) {
+    if (nsegs <= 0) return;
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((nsegs + 3) / 4)), dim3(256), 0, s, a, segs, nsegs, list, counts);
 }
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -38,6 +39,7 @@ int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bo
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
+void launch_compact(const FockArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
 void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s);
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
                     const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
@@ -129,12 +131,30 @@ struct ClassHost {
     long long total32 = 0;       // of which in the FP32 lists
     DBuf<Tile> d_tiles, d_tiles32;
     DBuf<int2> d_runs, d_runs32;
+    // the same candidates as segments {bra pair, first ket, end ket, offset}
+    // (one bra pair and one ket group each), for the per-build density
+    // compaction; segruns: [first, end) segment of each bra pair
+    std::vector<int4> segs, segs32;
+    std::vector<int2> segruns, segruns32;
+    DBuf<int4> d_segs, d_segs32;
+    DBuf<int2> d_segruns, d_segruns32;
+    DBuf<int> d_list, d_list32, d_counts, d_counts32;
+    // compaction pays where segments span several 64-ket steps (with short
+    // segments every step is already the whole segment)
+    bool compact = false, compact32 = false;
 };
 
 static void tile_runs(const std::vector<Tile>& T, std::vector<int2>& runs) {
     runs.clear();
     for (size_t k = 0; k < T.size(); ++k)
         if (runs.empty() || T[runs.back().x].p != T[k].p) runs.push_back({(int)k, (int)k + 1});
+        else runs.back().y = (int)k + 1;
+}
+
+static void seg_runs(const std::vector<int4>& S, std::vector<int2>& runs) {
+    runs.clear();
+    for (size_t k = 0; k < S.size(); ++k)
+        if (runs.empty() || S[runs.back().x].x != S[k].x) runs.push_back({(int)k, (int)k + 1});
         else runs.back().y = (int)k + 1;
 }
 
@@ -546,7 +566,7 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         ClassHost& CH = c->classes[k];
-        long long total = 0, total32 = 0;
+        long long total = 0, total32 = 0, seg_off = 0, seg_off32 = 0;
         for (size_t gb = 0; gb < groups.size(); ++gb) {
             if (groups[gb].cls != cb) continue;
             const Group& G1 = groups[gb];
@@ -582,6 +602,16 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                         CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo64 - q), 0});
                     for (int q = lo64; q < lo; q += chunk)
                         CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
+                    if (!fock_class_is_team(k)) {
+                        if (lo64 > q_begin) {
+                            CH.segs.push_back({G1.start + i, G2.start + q_begin, G2.start + lo64, (int)seg_off});
+                            seg_off += lo64 - q_begin;
+                        }
+                        if (lo > lo64) {
+                            CH.segs32.push_back({G1.start + i, G2.start + lo64, G2.start + lo, (int)seg_off32});
+                            seg_off32 += lo - lo64;
+                        }
+                    }
                     if (lo > q_begin) total += lo - q_begin;
                     if (lo > lo64) total32 += lo - lo64;
                 }
@@ -589,6 +619,12 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         }
         tile_runs(CH.tiles, CH.runs);
         tile_runs(CH.tiles32, CH.runs32);
+        seg_runs(CH.segs, CH.segruns);
+        seg_runs(CH.segs32, CH.segruns32);
+        if (seg_off > INT32_MAX || seg_off32 > INT32_MAX) {
+            set_err("class %d: more than 2^31 candidate quartets", k);
+            return -1;
+        }
         CH.total = total;
         CH.total32 = total32;
     }
@@ -607,10 +643,26 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     }
     c->pair_Y = Yp;
     c->grad_prepared = false;
-    for (auto& CH : c->classes)
+    for (auto& CH : c->classes) {
         if (CH.d_tiles.upload(CH.tiles) || CH.d_tiles32.upload(CH.tiles32) || CH.d_runs.upload(CH.runs) ||
             CH.d_runs32.upload(CH.runs32))
             return -1;
+        size_t nl = 0, nl32 = 0;
+        for (const int4& g : CH.segs) nl += (size_t)(g.z - g.y);
+        for (const int4& g : CH.segs32) nl32 += (size_t)(g.z - g.y);
+        // compaction costs about as much per candidate as an FP32 (ss|ss), (ps|ss)
+        // or (ps|ps) quartet saves, and more than an FP64 (ss|ss) one (measured on
+        // (H2O)32); it pays for every other class
+        const int k = (int)(&CH - c->classes.data());
+        const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
+        const bool small = kClassL[k][1] == 0 && kClassL[k][3] == 0 && LT <= 2 && kClassL[k][0] <= 1;
+        CH.compact = !CH.segs.empty() && LT >= 1;
+        CH.compact32 = !CH.segs32.empty() && !small;
+        if (CH.d_segs.upload(CH.segs) || CH.d_segs32.upload(CH.segs32) || CH.d_segruns.upload(CH.segruns) ||
+            CH.d_segruns32.upload(CH.segruns32) || CH.d_list.alloc(nl) || CH.d_list32.alloc(nl32) ||
+            CH.d_counts.alloc(CH.segs.size()) || CH.d_counts32.alloc(CH.segs32.size()))
+            return -1;
+    }
     if (mixed) {
         auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
         if (c->dkf_zeta.upload(f32(kz)) || c->dkf_oo2z.upload(f32(ko)) || c->dkf_Px.upload(f32(kx)) ||
@@ -728,7 +780,14 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 long long cap = 256LL * 16;
                 const size_t scratch = fock_class_scratch(k);
                 if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
-                if (use_runs) {
+                if (use_runs && CH.compact) {
+                    // density screen first: every lane of the class kernel then has a quartet
+                    launch_compact(a, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
+                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
+                                (long long)CH.segruns.size(), CH.d_segs.p, CH.d_list.p, CH.d_counts.p};
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), rows_lds,
+                                      s);
+                } else if (use_runs) {
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_runs.p, (long long)CH.runs.size()};
                     launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), rows_lds, s);
                 } else {
@@ -746,18 +805,27 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             long long cap = 256LL * 16;
             const size_t scratch = fock_class_scratch_f32(k);
             if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+            const int slot = kNumQuartetClasses + k;
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
             ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
             int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
             size_t lds = 0;
             if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 16);
-            if (use_runs) {
+            if (use_runs && CH.compact32) {
+                launch_compact(a, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
+                w.runs = CH.d_segruns32.p;
+                w.nruns = (long long)CH.segruns32.size();
+                w.segs = CH.d_segs32.p;
+                w.list = CH.d_list32.p;
+                w.counts = CH.d_counts32.p;
+                grid = (int)std::min<long long>(w.nruns, cap);
+                lds = rows_lds;
+            } else if (use_runs) {
                 w.runs = CH.d_runs32.p;
                 w.nruns = (long long)CH.runs32.size();
                 grid = (int)std::min<long long>(w.nruns, cap);
                 lds = rows_lds;
             }
-            const int slot = kNumQuartetClasses + k;
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
             if (fock_class_is_team(k)) launch_team_f32_class(k, a, w, grid, s);
             else launch_fock_f32_class(k, a, w, grid, lds, s);
             HIPCHK(hipGetLastError());

@@ -74,6 +74,13 @@ struct ClassWork {
     // them to X once per run
     const int2* runs;
     long long nruns;
+    // optional (with runs): the density-screened quartets compacted per
+    // segment (one bra pair, one ket group): segs[s] = {bra pair, first ket,
+    // end ket, offset}, the surviving kets of segment s at list[offset ..
+    // offset + counts[s]); runs then index segments
+    const int4* segs;
+    const int* list;
+    const int* counts;
 };
 
 // FP32 copy of the kernels' primitive-pair data (same layout as DevPrims).

@@ -276,6 +276,8 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const bool runs = w.runs != nullptr;
     unsigned long long* const xr = runs ? s_xrows : nullptr;
     const int wib = threadIdx.x >> 6;
+    const bool compact = runs && w.segs != nullptr;
+    int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
     for (;;) {
     long long t0, t1, tstep;
@@ -284,17 +286,45 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         if (run >= w.nruns) break;
         const int2 rg = w.runs[run];
         t0 = rg.x + wib; t1 = rg.y; tstep = 4;
-        rp = w.tiles[rg.x].p;
+        rp = compact ? w.segs[rg.x].x : w.tiles[rg.x].p;
+        c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
         __syncthreads();
     } else {
         t0 = wave0; t1 = w.ntiles; tstep = nwaves;
     }
-    for (long long t = t0; t < t1; t += tstep) {
-        const Tile tl = w.tiles[t];
-        const int p = __builtin_amdgcn_readfirstlane(tl.p);
-        const int q = tl.q0 + lane;
-        const bool inrange = lane < tl.len;
+    for (long long t = t0;; t += tstep) {
+        int p, q;
+        bool inrange;
+        if (compact) {
+            // the run's segments in steps of 64 surviving kets; step g of the
+            // run goes to wavefront g % 4 (all of this is wavefront-uniform)
+            bool found = false;
+            while (c_seg < c_end) {
+                if (c_nst < 0) c_nst = (w.counts[c_seg] + 63) >> 6;
+                if (c_step < c_nst) {
+                    const bool mine = (c_gt & 3) == wib;
+                    ++c_gt;
+                    if (mine) { found = true; break; }
+                    ++c_step;
+                    continue;
+                }
+                ++c_seg; c_step = 0; c_nst = -1;
+            }
+            if (!found) break;
+            const int4 sg = w.segs[c_seg];
+            const int idx = c_step * 64 + lane;
+            p = __builtin_amdgcn_readfirstlane(sg.x);
+            inrange = idx < w.counts[c_seg];
+            q = inrange ? w.list[sg.w + idx] : 0;
+            ++c_step;
+        } else {
+            if (t >= t1) break;
+            const Tile tl = w.tiles[t];
+            p = __builtin_amdgcn_readfirstlane(tl.p);
+            q = tl.q0 + lane;
+            inrange = lane < tl.len;
+        }
 
         const int qa = a.pr.qa[p], qb = a.pr.qb[p];
         const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
@@ -588,6 +618,8 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const bool runs = w.runs != nullptr;
     unsigned long long* const xr = runs ? s_xrows : nullptr;
     const int wib = threadIdx.x >> 6;
+    const bool compact = runs && w.segs != nullptr;
+    int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
     for (;;) {
     long long t0, t1, tstep;
@@ -596,17 +628,45 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
         if (run >= w.nruns) break;
         const int2 rg = w.runs[run];
         t0 = rg.x + wib; t1 = rg.y; tstep = 4;
-        rp = w.tiles[rg.x].p;
+        rp = compact ? w.segs[rg.x].x : w.tiles[rg.x].p;
+        c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
         __syncthreads();
     } else {
         t0 = wave0; t1 = w.ntiles; tstep = nwaves;
     }
-    for (long long t = t0; t < t1; t += tstep) {
-        const Tile tl = w.tiles[t];
-        const int p = __builtin_amdgcn_readfirstlane(tl.p);
-        const int q = tl.q0 + lane;
-        const bool inrange = lane < tl.len;
+    for (long long t = t0;; t += tstep) {
+        int p, q;
+        bool inrange;
+        if (compact) {
+            // the run's segments in steps of 64 surviving kets; step g of the
+            // run goes to wavefront g % 4 (all of this is wavefront-uniform)
+            bool found = false;
+            while (c_seg < c_end) {
+                if (c_nst < 0) c_nst = (w.counts[c_seg] + 63) >> 6;
+                if (c_step < c_nst) {
+                    const bool mine = (c_gt & 3) == wib;
+                    ++c_gt;
+                    if (mine) { found = true; break; }
+                    ++c_step;
+                    continue;
+                }
+                ++c_seg; c_step = 0; c_nst = -1;
+            }
+            if (!found) break;
+            const int4 sg = w.segs[c_seg];
+            const int idx = c_step * 64 + lane;
+            p = __builtin_amdgcn_readfirstlane(sg.x);
+            inrange = idx < w.counts[c_seg];
+            q = inrange ? w.list[sg.w + idx] : 0;
+            ++c_step;
+        } else {
+            if (t >= t1) break;
+            const Tile tl = w.tiles[t];
+            p = __builtin_amdgcn_readfirstlane(tl.p);
+            q = tl.q0 + lane;
+            inrange = lane < tl.len;
+        }
         const int qa = a.pr.qa[p], qb = a.pr.qb[p];
         const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
         bool active = false;
