@@ -27,32 +27,54 @@ namespace qk {
 // helper kernels
 
 // Density screen of the candidate quartets, compacted: one wavefront per
-// segment (a bra pair and a ket range of one group) tests Y_p Y_q DNmax > tau
-// for its candidate kets 64 at a time and appends the survivors in order
-// (ballot + prefix count) at list[segs[s].w ...]; counts[s] gets their number.
-// The class kernels then fill all 64 lanes with surviving quartets.
-__global__ void __launch_bounds__(256) compact_kernel(const FockArgs a, const int4* __restrict__ segs, long long nsegs,
-                                                      int* __restrict__ list, int* __restrict__ counts) {
+// segment (a bra pair and a ket range of one group) tests its candidate kets
+// 64 at a time and appends the survivors in order (ballot + prefix count) at
+// list[segs[s].w ...]; counts[s] gets their number.  The class kernels then
+// fill all 64 lanes with surviving quartets.  The test is the class kernels'
+// own, same operation order: Fock Y_p Y_q DNmax > tau; gradient (one-centre
+// quartets skipped) Y_p Y_q > tau and Y_p Y_q DNmax > tau.  With a.steps set,
+// the survivors are also cut into steps of 64 and queued (the gradient's light
+// kernels stride over that queue).
+__global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const int4* __restrict__ segs,
+                                                      long long nsegs, int* __restrict__ list, int* __restrict__ counts) {
     const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (s >= nsegs) return;   // the whole wavefront
     const int4 sg = segs[s];
-    const int qa = a.pr.qa[sg.x], qb = a.pr.qb[sg.x];
-    const int sA = a.q.shell[qa], sB = a.q.shell[qb];
-    const double yp = a.pr.Y[sg.x];
+    const int qa = a.qa[sg.x], qb = a.qb[sg.x];
+    const int sA = a.shell[qa], sB = a.shell[qb];
+    const int atA = a.atom[qa], atB = a.atom[qb];
+    const double yp = a.Y[sg.x];
     int cnt = 0;
     for (int q0 = sg.y; q0 < sg.z; q0 += 64) {
         const int q = q0 + lane;
         bool keep = false;
         if (q < sg.z) {
-            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, a.q.shell[a.pr.qa[q]], a.q.shell[a.pr.qb[q]]);
-            keep = a.pr.Y[q] * yp * dn > a.tau;   // the class kernels' test, same operation order
+            const int qc = a.qa[q], qd = a.qb[q];
+            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, a.shell[qc], a.shell[qd]);
+            if (a.grad) {
+                const int atC = a.atom[qc], atD = a.atom[qd];
+                const double ct = a.Y[q] * yp;
+                keep = !(atA == atB && atA == atC && atA == atD) && ct > a.tau && ct * dn > a.tau;
+            } else {
+                keep = a.Y[q] * yp * dn > a.tau;
+            }
         }
         const unsigned long long bal = __ballot(keep);
         if (keep) list[sg.w + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = q;
         cnt += __popcll(bal);
     }
     if (lane == 0) counts[s] = cnt;
+    if (a.steps != nullptr && cnt > 0) {
+        // this segment's steps, appended to the queue (its order varies from
+        // run to run; the fixed-point sums do not depend on it)
+        const int nst = (cnt + 63) >> 6;
+        int base = 0;
+        if (lane == 0) base = (int)atomicAdd(a.nsteps, (unsigned long long)nst);
+        base = __shfl(base, 0);
+        for (int j = lane; j < nst; j += 64)
+            a.steps[base + j] = Tile{sg.x, sg.w + 64 * j, min(64, cnt - 64 * j), 0};
+    }
 }
 
 // Dp = diag(cons) D diag(cons): folds QUICK's per-function cons factors
@@ -203,7 +225,7 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
     hipLaunchKernelGGL(density_cutoff, dim3(blocks_for((long long)ns * ns, 256)), dim3(256), 0, s, D, sh_f0, sh_nf,
                        cm, n, ns);
 }
-void launch_compact(const FockArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s) {
+void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s) {
     if (nsegs <= 0) return;
     hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((nsegs + 3) / 4)), dim3(256), 0, s, a, segs, nsegs, list, counts);
 }

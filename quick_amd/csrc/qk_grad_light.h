@@ -6,7 +6,10 @@
 // weighting as the table-driven kernel in qk_grad.hip; the derivative sums of
 // every quartet are converted to int64 fixed point (2^-44) and added into a
 // per-block LDS copy of the gradient, flushed once per block (or straight into
-// global memory when the gradient does not fit in LDS).
+// global memory when the gradient does not fit in LDS).  The work comes either
+// as tiles (a bra pair and up to 64 consecutive kets) or, as the host launches
+// it, compacted by compact_kernel to the quartets that pass the screen, so
+// that every lane of a step holds a surviving quartet.
 #pragma once
 #include "qk_kernels.h"
 #include "grad_light.inc"
@@ -132,11 +135,16 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
         for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x) s_grad[k] = 0ull;
     __syncthreads();
 
-    for (long long t = wave0; t < w.ntiles; t += nwaves) {
-        const Tile tl = w.tiles[t];
+    // compacted work (w.steps set): the queue of steps written by compact_kernel,
+    // each a bra pair and up to 64 surviving kets in w.list; otherwise tiles of
+    // a bra pair and consecutive kets
+    const bool compact = w.steps != nullptr;
+    const long long nwork = compact ? (long long)*w.nsteps : w.ntiles;
+    for (long long t = wave0; t < nwork; t += nwaves) {
+        const Tile tl = compact ? w.steps[t] : w.tiles[t];
         const int p = __builtin_amdgcn_readfirstlane(tl.p);
-        const int q = tl.q0 + lane;
         const bool inrange = lane < tl.len;
+        const int q = compact ? (inrange ? w.list[tl.q0 + lane] : 0) : tl.q0 + lane;
         const int qa = a.pr.qa[p], qb = a.pr.qb[p];
         const int qc = inrange ? a.pr.qa[q] : 0, qd = inrange ? a.pr.qb[q] : 0;
         const int atA = a.q.atom[qa], atB = a.q.atom[qb], atC = a.q.atom[qc], atD = a.q.atom[qd];

@@ -39,7 +39,7 @@ int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bo
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
-void launch_compact(const FockArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
+void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
 void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s);
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
                     const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
@@ -222,6 +222,15 @@ struct qk_ctx {
     DBuf<int> d_gtab[kNumQuartetClasses];
     std::vector<Tile> gtiles[kNumQuartetClasses], gtiles32[kNumQuartetClasses];   // FP64 / FP32 work
     DBuf<Tile> d_gtiles[kNumQuartetClasses], d_gtiles32[kNumQuartetClasses];
+    // light classes: segments {bra pair, first ket, end ket, list offset} of the
+    // per-gradient density compaction and their runs per bra pair (index 0 FP64,
+    // 1 FP32)
+    std::vector<int4> gsegs[2][kNumQuartetClasses];
+    long long gsteps_max[2][kNumQuartetClasses] = {};   // bound on the queued steps
+    DBuf<int4> d_gsegs[2][kNumQuartetClasses];
+    DBuf<Tile> d_gsteps[2][kNumQuartetClasses];
+    DBuf<int> d_glist[2][kNumQuartetClasses], d_gcounts[2][kNumQuartetClasses];
+    DBuf<unsigned long long> d_gnsteps;   // step counters, 2 x classes
     double grad_prep_cutoff = -1, grad_prep_thr = -1;
     int grad_prep_precision = -1;
     bool grad_prepared = false;
@@ -718,6 +727,23 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     return a;
 }
 
+// inputs of the density compaction: the Fock test (grad 0) or the gradient's
+static CompactArgs compact_args(const FockArgs& a, int grad, double tau) {
+    CompactArgs ca;
+    ca.qa = a.pr.qa;
+    ca.qb = a.pr.qb;
+    ca.shell = a.q.shell;
+    ca.atom = a.q.atom;
+    ca.Y = a.pr.Y;
+    ca.cutmat = a.cutmat;
+    ca.nshell = a.nshell;
+    ca.tau = tau;
+    ca.grad = grad;
+    ca.steps = nullptr;
+    ca.nsteps = nullptr;
+    return ca;
+}
+
 // folds the previous launches' kernel times into the running sums (waits for them)
 static int fold_timing(qk_ctx* c) {
     for (int k = 0; k < qk_ctx::kSlots; ++k) {
@@ -748,6 +774,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);
+    const CompactArgs ca = compact_args(a, 0, a.tau);
     if (fold_timing(c)) return -1;
     // heaviest classes first, round-robin over the streams
     std::vector<int> order(kNumQuartetClasses);
@@ -782,7 +809,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
                 if (use_runs && CH.compact) {
                     // density screen first: every lane of the class kernel then has a quartet
-                    launch_compact(a, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
+                    launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
                                 (long long)CH.segruns.size(), CH.d_segs.p, CH.d_list.p, CH.d_counts.p};
                     launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), rows_lds,
@@ -812,7 +839,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             size_t lds = 0;
             if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 16);
             if (use_runs && CH.compact32) {
-                launch_compact(a, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
+                launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
                 w.runs = CH.d_segruns32.p;
                 w.nruns = (long long)CH.segruns32.size();
                 w.segs = CH.d_segs32.p;
@@ -1084,7 +1111,13 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
         std::vector<Tile>& T32 = c->gtiles32[k];
         T.clear();
         T32.clear();
+        const bool light = grad_class_is_light(k);
+        c->gsegs[0][k].clear();
+        c->gsegs[1][k].clear();
+        long long off[2] = {0, 0}, nst[2] = {0, 0};
         const size_t ng = c->grp_cls.size();
+        // (bra group, ket group, bra) order: neighbouring wavefronts read the
+        // same kets (measured faster than bra-major order for the gradient)
         for (size_t gb = 0; gb < ng; ++gb) {
             if (c->grp_cls[gb] != cb) continue;
             for (size_t gk = 0; gk < ng; ++gk) {
@@ -1114,17 +1147,58 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
                         T.push_back({s1 + i, s2 + q, std::min(chunk, lo64 - q), 0});
                     for (int q = lo64; q < lo; q += chunk)
                         T32.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
+                    if (light && lo64 > q_begin) {
+                        c->gsegs[0][k].push_back({s1 + i, s2 + q_begin, s2 + lo64, (int)off[0]});
+                        off[0] += lo64 - q_begin;
+                        nst[0] += (lo64 - q_begin + 63) / 64;
+                    }
+                    if (light && lo > lo64) {
+                        c->gsegs[1][k].push_back({s1 + i, s2 + lo64, s2 + lo, (int)off[1]});
+                        off[1] += lo - lo64;
+                        nst[1] += (lo - lo64 + 63) / 64;
+                    }
                 }
             }
         }
+        if (off[0] > INT32_MAX || off[1] > INT32_MAX) {
+            set_err("gradient class %d: more than 2^31 candidate quartets", k);
+            return -1;
+        }
         if (c->d_gtiles[k].upload(T) || c->d_gtiles32[k].upload(T32)) return -1;
+        for (int f = 0; f < 2; ++f) {
+            c->gsteps_max[f][k] = nst[f];
+            if (c->d_gsegs[f][k].upload(c->gsegs[f][k]) || c->d_gsteps[f][k].alloc((size_t)std::max(nst[f], 1LL)) ||
+                c->d_glist[f][k].alloc((size_t)std::max(off[f], 1LL)) ||
+                c->d_gcounts[f][k].alloc(std::max<size_t>(c->gsegs[f][k].size(), 1)))
+                return -1;
+        }
     }
-    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses)) return -1;
+    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses) ||
+        c->d_gnsteps.alloc(2 * kNumQuartetClasses))
+        return -1;
     c->grad_prep_cutoff = cutoff;
     c->grad_prep_precision = o->precision;
     c->grad_prep_thr = o->fp32_threshold;
     c->grad_prepared = true;
     return 0;
+}
+
+// Whether the light class k's list is density-compacted before its kernel.
+// The compaction pass costs about one gather-heavy test per candidate quartet;
+// it pays where lanes are expensive and many candidates fail the density test.
+// Measured per class on (H2O)32 (compaction everywhere against nowhere): every
+// light class of an FP64 gradient gains or is even (dpps 17.1 -> 12.7 ms, pppp
+// 17.6 -> 14.4 ms, ssss even); in a mixed gradient the FP64 list holds the
+// short ket ranges of the strongest pairs, where pppp (3.7 vs 3.8 ms) and ssss
+// (1.16 -> 0.99 ms) gain and the other light classes lose up to 0.35 ms
+// (psss); of the FP32 lists those of total angular momentum >= 3 gain (dpps
+// 2.81 -> 2.45 ms) while psss, ssss and ppss lose.
+static bool grad_compacts(int k, bool fp32, bool mixed) {
+    const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
+    if (fp32) return LT >= 3;
+    if (!mixed) return true;
+    return LT == 0 || ncart_of(kClassL[k][0]) * ncart_of(kClassL[k][1]) * ncart_of(kClassL[k][2]) *
+                              ncart_of(kClassL[k][3]) >= 81;
 }
 
 // dgrad <- dgrad + dE2/dR for density dD (cutmatrix from dD unless given)
@@ -1143,6 +1217,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
     launch_fill_u64(c->d_gacc.p, 3LL * c->natom, s);
     launch_fill_u64(c->d_gcounters.p, 4LL * kNumQuartetClasses, s);
+    launch_fill_u64(c->d_gnsteps.p, 2LL * kNumQuartetClasses, s);
     HIPCHK(hipGetLastError());
     GradArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
@@ -1165,6 +1240,18 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     a.ebf = c->dkf_eb.p;
     a.Dpf = c->d_Dpf.p;
     a.boys_tab = c->d_boys_tab.p;
+    CompactArgs ca;   // the gradient's screen (one-centre skip, Y Y > cutoff, Y Y DNmax > cutoff)
+    ca.qa = a.pr.qa;
+    ca.qb = a.pr.qb;
+    ca.shell = a.q.shell;
+    ca.atom = a.q.atom;
+    ca.Y = a.pr.Y;
+    ca.cutmat = a.cutmat;
+    ca.nshell = ns;
+    ca.tau = o->tau;
+    ca.grad = 1;
+    ca.steps = nullptr;
+    ca.nsteps = nullptr;
     for (int pass = 0; pass < (mixed ? 2 : 1); ++pass) {
         const bool fp32 = pass == 1;
         for (int k = 0; k < kNumQuartetClasses; ++k) {
@@ -1174,7 +1261,23 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             a.counters = c->d_gcounters.p + 2 * (fp32 ? kNumQuartetClasses + k : k);
             const int slot = (fp32 ? 3 : 2) * kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
-            if (grad_class_is_light(k)) {
+            if (grad_class_is_light(k) && grad_compacts(k, fp32, mixed)) {
+                // density compaction, then the kernel over the queued steps
+                const int f = fp32 ? 1 : 0;
+                CompactArgs cq = ca;
+                cq.steps = c->d_gsteps[f][k].p;
+                cq.nsteps = c->d_gnsteps.p + f * kNumQuartetClasses + k;
+                launch_compact(cq, c->d_gsegs[f][k].p, (long long)c->gsegs[f][k].size(), c->d_glist[f][k].p,
+                               c->d_gcounts[f][k].p, s);
+                w.list = c->d_glist[f][k].p;
+                w.steps = cq.steps;
+                w.nsteps = cq.nsteps;
+                const int grid = (int)std::min<long long>((c->gsteps_max[f][k] + 3) / 4, 256LL * 8);
+                if (launch_grad_light(k, a, w, grid, fp32, s)) {
+                    set_err("gradient kernel launch failed for class %d", k);
+                    return -1;
+                }
+            } else if (grad_class_is_light(k)) {
                 const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
                 if (launch_grad_light(k, a, w, grid, fp32, s)) {
                     set_err("gradient kernel launch failed for class %d", k);

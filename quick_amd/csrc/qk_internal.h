@@ -81,6 +81,28 @@ struct ClassWork {
     const int4* segs;
     const int* list;
     const int* counts;
+    // optional (without runs): the steps of compact_kernel's queue, tiles whose
+    // kets are list[q0 .. q0 + len); their number is *nsteps
+    const Tile* steps;
+    const unsigned long long* nsteps;
+};
+
+// Inputs of the per-build density compaction (compact_kernel): the pair
+// lists, shells and atoms of the Qshells, Schwarz bounds and density screen.
+struct CompactArgs {
+    const int* qa;
+    const int* qb;
+    const int* shell;
+    const int* atom;
+    const double* Y;
+    const double* cutmat;
+    int nshell;
+    double tau;
+    int grad;   // 1: the gradient's test (one-centre skip, Y Y > tau and Y Y DNmax > tau)
+    // optional: a queue of work steps, each a bra pair and up to 64 surviving
+    // kets of one segment ({bra, list offset, count}); nsteps counts them
+    Tile* steps;
+    unsigned long long* nsteps;
 };
 
 // FP32 copy of the kernels' primitive-pair data (same layout as DevPrims).
