@@ -33,8 +33,9 @@ namespace qk {
 // fill all 64 lanes with surviving quartets.  The test is the class kernels'
 // own, same operation order: Fock Y_p Y_q DNmax > tau; gradient (one-centre
 // quartets skipped) Y_p Y_q > tau and Y_p Y_q DNmax > tau.  With a.steps set,
-// the survivors are also cut into steps of 64 and queued (the gradient's light
-// kernels stride over that queue).
+// the survivors are also cut into steps of 64, written to the slots of the
+// segment's tiles (same order as the tiles; the gradient's light kernels
+// stride over them).
 __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const int4* __restrict__ segs,
                                                       long long nsegs, int* __restrict__ list, int* __restrict__ counts) {
     const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -65,15 +66,11 @@ __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const
         cnt += __popcll(bal);
     }
     if (lane == 0) counts[s] = cnt;
-    if (a.steps != nullptr && cnt > 0) {
-        // this segment's steps, appended to the queue (its order varies from
-        // run to run; the fixed-point sums do not depend on it)
-        const int nst = (cnt + 63) >> 6;
-        int base = 0;
-        if (lane == 0) base = (int)atomicAdd(a.nsteps, (unsigned long long)nst);
-        base = __shfl(base, 0);
-        for (int j = lane; j < nst; j += 64)
-            a.steps[base + j] = Tile{sg.x, sg.w + 64 * j, min(64, cnt - 64 * j), 0};
+    if (a.steps != nullptr) {
+        // this segment's slots: its steps of 64 survivors, then empty ones
+        const int nslot = (sg.z - sg.y + 63) >> 6, base = a.slot0[s];
+        for (int j = lane; j < nslot; j += 64)
+            a.steps[base + j] = Tile{sg.x, sg.w + 64 * j, max(0, min(64, cnt - 64 * j)), 0};
     }
 }
 

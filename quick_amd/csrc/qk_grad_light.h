@@ -8,8 +8,8 @@
 // per-block LDS copy of the gradient, flushed once per block (or straight into
 // global memory when the gradient does not fit in LDS).  The work comes either
 // as tiles (a bra pair and up to 64 consecutive kets) or, as the host launches
-// it, compacted by compact_kernel to the quartets that pass the screen, so
-// that every lane of a step holds a surviving quartet.
+// it, compacted by compact_kernel to the quartets that pass the screen, packed
+// 64 to a step in the tiles' places.
 #pragma once
 #include "qk_kernels.h"
 #include "grad_light.inc"
@@ -135,13 +135,13 @@ __global__ void __launch_bounds__(256, (GradLightOcc<LA, LB, LC, LD, R>::value))
         for (int k = threadIdx.x; k < 3 * a.natom; k += blockDim.x) s_grad[k] = 0ull;
     __syncthreads();
 
-    // compacted work (w.steps set): the queue of steps written by compact_kernel,
-    // each a bra pair and up to 64 surviving kets in w.list; otherwise tiles of
-    // a bra pair and consecutive kets
+    // compacted work (w.steps set): the steps written by compact_kernel in the
+    // tiles' places, each a bra pair and up to 64 surviving kets in w.list;
+    // otherwise tiles of a bra pair and consecutive kets
     const bool compact = w.steps != nullptr;
-    const long long nwork = compact ? (long long)*w.nsteps : w.ntiles;
-    for (long long t = wave0; t < nwork; t += nwaves) {
+    for (long long t = wave0; t < w.ntiles; t += nwaves) {
         const Tile tl = compact ? w.steps[t] : w.tiles[t];
+        if (tl.len == 0) continue;   // wavefront-uniform: no survivor in this slot
         const int p = __builtin_amdgcn_readfirstlane(tl.p);
         const bool inrange = lane < tl.len;
         const int q = compact ? (inrange ? w.list[tl.q0 + lane] : 0) : tl.q0 + lane;

@@ -226,11 +226,11 @@ struct qk_ctx {
     // per-gradient density compaction and their runs per bra pair (index 0 FP64,
     // 1 FP32)
     std::vector<int4> gsegs[2][kNumQuartetClasses];
-    long long gsteps_max[2][kNumQuartetClasses] = {};   // bound on the queued steps
+    std::vector<int> gslot0[2][kNumQuartetClasses];   // first tile of each segment
     DBuf<int4> d_gsegs[2][kNumQuartetClasses];
     DBuf<Tile> d_gsteps[2][kNumQuartetClasses];
+    DBuf<int> d_gslot0[2][kNumQuartetClasses];
     DBuf<int> d_glist[2][kNumQuartetClasses], d_gcounts[2][kNumQuartetClasses];
-    DBuf<unsigned long long> d_gnsteps;   // step counters, 2 x classes
     double grad_prep_cutoff = -1, grad_prep_thr = -1;
     int grad_prep_precision = -1;
     bool grad_prepared = false;
@@ -740,7 +740,7 @@ static CompactArgs compact_args(const FockArgs& a, int grad, double tau) {
     ca.tau = tau;
     ca.grad = grad;
     ca.steps = nullptr;
-    ca.nsteps = nullptr;
+    ca.slot0 = nullptr;
     return ca;
 }
 
@@ -1112,9 +1112,11 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
         T.clear();
         T32.clear();
         const bool light = grad_class_is_light(k);
-        c->gsegs[0][k].clear();
-        c->gsegs[1][k].clear();
-        long long off[2] = {0, 0}, nst[2] = {0, 0};
+        long long off[2] = {0, 0};
+        for (int f = 0; f < 2; ++f) {
+            c->gsegs[f][k].clear();
+            c->gslot0[f][k].clear();
+        }
         const size_t ng = c->grp_cls.size();
         // (bra group, ket group, bra) order: neighbouring wavefronts read the
         // same kets (measured faster than bra-major order for the gradient)
@@ -1143,39 +1145,39 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
                         }
                         lo64 = std::max(a0, q_begin);
                     }
+                    // a light class's segment owns the slots of its tiles (64 kets each)
+                    if (light && lo64 > q_begin) {
+                        c->gsegs[0][k].push_back({s1 + i, s2 + q_begin, s2 + lo64, (int)off[0]});
+                        c->gslot0[0][k].push_back((int)T.size());
+                        off[0] += lo64 - q_begin;
+                    }
+                    if (light && lo > lo64) {
+                        c->gsegs[1][k].push_back({s1 + i, s2 + lo64, s2 + lo, (int)off[1]});
+                        c->gslot0[1][k].push_back((int)T32.size());
+                        off[1] += lo - lo64;
+                    }
                     for (int q = q_begin; q < lo64; q += chunk)
                         T.push_back({s1 + i, s2 + q, std::min(chunk, lo64 - q), 0});
                     for (int q = lo64; q < lo; q += chunk)
                         T32.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
-                    if (light && lo64 > q_begin) {
-                        c->gsegs[0][k].push_back({s1 + i, s2 + q_begin, s2 + lo64, (int)off[0]});
-                        off[0] += lo64 - q_begin;
-                        nst[0] += (lo64 - q_begin + 63) / 64;
-                    }
-                    if (light && lo > lo64) {
-                        c->gsegs[1][k].push_back({s1 + i, s2 + lo64, s2 + lo, (int)off[1]});
-                        off[1] += lo - lo64;
-                        nst[1] += (lo - lo64 + 63) / 64;
-                    }
                 }
             }
         }
-        if (off[0] > INT32_MAX || off[1] > INT32_MAX) {
+        if (off[0] > INT32_MAX || off[1] > INT32_MAX || T.size() > INT32_MAX || T32.size() > INT32_MAX) {
             set_err("gradient class %d: more than 2^31 candidate quartets", k);
             return -1;
         }
         if (c->d_gtiles[k].upload(T) || c->d_gtiles32[k].upload(T32)) return -1;
         for (int f = 0; f < 2; ++f) {
-            c->gsteps_max[f][k] = nst[f];
-            if (c->d_gsegs[f][k].upload(c->gsegs[f][k]) || c->d_gsteps[f][k].alloc((size_t)std::max(nst[f], 1LL)) ||
+            const size_t nslot = light ? (f ? T32.size() : T.size()) : 0;
+            if (c->d_gsegs[f][k].upload(c->gsegs[f][k]) || c->d_gslot0[f][k].upload(c->gslot0[f][k]) ||
+                c->d_gsteps[f][k].alloc(std::max<size_t>(nslot, 1)) ||
                 c->d_glist[f][k].alloc((size_t)std::max(off[f], 1LL)) ||
                 c->d_gcounts[f][k].alloc(std::max<size_t>(c->gsegs[f][k].size(), 1)))
                 return -1;
         }
     }
-    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses) ||
-        c->d_gnsteps.alloc(2 * kNumQuartetClasses))
-        return -1;
+    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses)) return -1;
     c->grad_prep_cutoff = cutoff;
     c->grad_prep_precision = o->precision;
     c->grad_prep_thr = o->fp32_threshold;
@@ -1217,7 +1219,6 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
     launch_fill_u64(c->d_gacc.p, 3LL * c->natom, s);
     launch_fill_u64(c->d_gcounters.p, 4LL * kNumQuartetClasses, s);
-    launch_fill_u64(c->d_gnsteps.p, 2LL * kNumQuartetClasses, s);
     HIPCHK(hipGetLastError());
     GradArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
@@ -1251,7 +1252,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     ca.tau = o->tau;
     ca.grad = 1;
     ca.steps = nullptr;
-    ca.nsteps = nullptr;
+    ca.slot0 = nullptr;
     for (int pass = 0; pass < (mixed ? 2 : 1); ++pass) {
         const bool fp32 = pass == 1;
         for (int k = 0; k < kNumQuartetClasses; ++k) {
@@ -1266,13 +1267,12 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 const int f = fp32 ? 1 : 0;
                 CompactArgs cq = ca;
                 cq.steps = c->d_gsteps[f][k].p;
-                cq.nsteps = c->d_gnsteps.p + f * kNumQuartetClasses + k;
+                cq.slot0 = c->d_gslot0[f][k].p;
                 launch_compact(cq, c->d_gsegs[f][k].p, (long long)c->gsegs[f][k].size(), c->d_glist[f][k].p,
                                c->d_gcounts[f][k].p, s);
                 w.list = c->d_glist[f][k].p;
                 w.steps = cq.steps;
-                w.nsteps = cq.nsteps;
-                const int grid = (int)std::min<long long>((c->gsteps_max[f][k] + 3) / 4, 256LL * 8);
+                const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
                 if (launch_grad_light(k, a, w, grid, fp32, s)) {
                     set_err("gradient kernel launch failed for class %d", k);
                     return -1;

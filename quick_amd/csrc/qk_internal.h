@@ -81,10 +81,10 @@ struct ClassWork {
     const int4* segs;
     const int* list;
     const int* counts;
-    // optional (without runs): the steps of compact_kernel's queue, tiles whose
-    // kets are list[q0 .. q0 + len); their number is *nsteps
+    // optional (without runs): the steps written by compact_kernel, one per
+    // tile and in tile order: tiles whose kets are list[q0 .. q0 + len)
+    // (len 0: nothing survived there)
     const Tile* steps;
-    const unsigned long long* nsteps;
 };
 
 // Inputs of the per-build density compaction (compact_kernel): the pair
@@ -99,10 +99,11 @@ struct CompactArgs {
     int nshell;
     double tau;
     int grad;   // 1: the gradient's test (one-centre skip, Y Y > tau and Y Y DNmax > tau)
-    // optional: a queue of work steps, each a bra pair and up to 64 surviving
-    // kets of one segment ({bra, list offset, count}); nsteps counts them
+    // optional: work steps, each a bra pair and up to 64 surviving kets of one
+    // segment ({bra, list offset, count}); segment s owns the ceil(len / 64)
+    // slots from slot0[s] (the slots of its tiles), the unused ones get count 0
     Tile* steps;
-    unsigned long long* nsteps;
+    const int* slot0;
 };
 
 // FP32 copy of the kernels' primitive-pair data (same layout as DevPrims).
