@@ -68,8 +68,11 @@ __device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, dou
 // QUICK's Boys fit; R = float (mixed precision, weak quartets): the tabulated
 // FP32 Boys expansion, FP32 LDS buffers and arithmetic, the nine sums promoted
 // to FP64 before the wavefront reduction.
+// At least three wavefronts per SIMD: the FP64 form then fits 168 VGPRs (76
+// bytes of spills) instead of 205 at two; measured on (H2O)32, FP64 gradient
+// 174.5 -> 172.4 ms (dpdp 14.3 -> 11.0 ms), mixed 91.4 -> 88.7 ms.
 template <int kGradTeam, typename R>
-__global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+__global__ void __launch_bounds__(kGradTeam, 3) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
     constexpr bool f64 = sizeof(R) == 8;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;          // thread of the block (kGradTeam threads share one quartet)
@@ -91,6 +94,9 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
     // a bra HRR term; pwc[64 + c] the same in C-D for the ket HRR
     R* pwc = Kv + NOUT;
     R* Fl = pwc + 128;
+    // the primitive quartet's VRR factors, indexed by side (0 bra, 1 ket) and
+    // direction: PA|QC (0-5), WP|WQ (6-11), 1/2zeta|1/2eta (12-13), rho/zeta|rho/eta (14-15)
+    R* geo = Fl + 16;
     const auto& PP = grad_prims<R>(a);
     const R* ea = grad_ea<R>(a);
     const R* eb = grad_eb<R>(a);
@@ -158,6 +164,13 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                     const R PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
                     const R QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
                     const R oo2ze = R(0.5) * inv;
+                    if (lane == 0) {
+                        geo[0] = PAx; geo[1] = PAy; geo[2] = PAz;
+                        geo[3] = QCx; geo[4] = QCy; geo[5] = QCz;
+                        geo[6] = WPx; geo[7] = WPy; geo[8] = WPz;
+                        geo[9] = WQx; geo[10] = WQy; geo[11] = WQz;
+                        geo[12] = oo2z; geo[13] = oo2e; geo[14] = roz; geo[15] = roe;
+                    }
                     __syncthreads();
                     for (int k = lev[0] + lane; k < lev[1]; k += kGradTeam) {
                         const uint4 nd = nodes[k];
@@ -180,13 +193,12 @@ __global__ void __launch_bounds__(kGradTeam) grad_kernel(const GradArgs a, const
                                 if (kb + u * kGradTeam >= k1) break;
                                 const uint4 nd = ndb[u];
                                 const unsigned fl = nd.w;
-                                const bool bra = (fl & 3u) == 1u;
-                                const int dir = (fl >> 2) & 3;
+                                const int side = (fl & 3u) == 1u ? 0 : 1;
+                                const int gi = side * 3 + int((fl >> 2) & 3u);
                                 const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
-                                const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
-                                const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
-                                const R c3 = n1 * (bra ? oo2z : oo2e);
-                                const R c4 = bra ? roz : roe;
+                                const R pa = geo[gi], wp = geo[6 + gi];
+                                const R c3 = n1 * geo[12 + side];
+                                const R c4 = geo[14 + side];
                                 const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
                                             c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
                                 V[nd.x & 0xffffu] = v;
@@ -349,12 +361,12 @@ __global__ void grad_finalize(const unsigned long long* __restrict__ G, double* 
 static int grad_team_of(const int* hdr) { return hdr[kGhSlots] >= 1800 ? 256 : 64; }
 
 // dynamic LDS of grad_kernel: the FP64 partial sums (9 x 4), then the VRR /
-// stage buffers, sets, density factors, powers and Boys values in the
+// stage buffers, sets, density factors, powers, Boys values and the VRR factors in the
 // kernel's precision
 size_t grad_lds_bytes(const int* hdr, bool fp32) {
     const int RV = hdr[kGhSlots] > hdr[kGhMaxStage] ? hdr[kGhSlots] : hdr[kGhMaxStage];
     const int nS = hdr[kGhN0] + hdr[kGhNA] + hdr[kGhNB] + hdr[kGhNC];
-    return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 128 + 16);
+    return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 128 + 16 + 16);
 }
 
 template <int TEAM, typename R>
