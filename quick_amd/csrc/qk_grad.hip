@@ -68,11 +68,14 @@ __device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, dou
 // QUICK's Boys fit; R = float (mixed precision, weak quartets): the tabulated
 // FP32 Boys expansion, FP32 LDS buffers and arithmetic, the nine sums promoted
 // to FP64 before the wavefront reduction.
-// At least three wavefronts per SIMD: the FP64 form then fits 168 VGPRs (76
-// bytes of spills) instead of 205 at two; measured on (H2O)32, FP64 gradient
-// 174.5 -> 172.4 ms (dpdp 14.3 -> 11.0 ms), mixed 91.4 -> 88.7 ms.
+// At least three wavefronts per SIMD in FP64, five in FP32: the FP64 form then
+// fits 168 VGPRs (76 bytes of spills) instead of 205 at two, the FP32 form 96
+// (no spills) instead of 100 at four.  Measured on (H2O)32: FP64 gradient
+// 174.5 -> 172.4 ms (dpdp 14.3 -> 11.0 ms); the mixed gradient's FP32 classes
+// gain 0.3-0.9 ms each at five (dpdp 6.7 -> 5.7 ms), except dppp, which the
+// host keeps at four resident wavefronts (16.0 ms at five against 13.0).
 template <int kGradTeam, typename R>
-__global__ void __launch_bounds__(kGradTeam, 3) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+__global__ void __launch_bounds__(kGradTeam, (sizeof(R) == 4 ? 5 : 3)) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
     constexpr bool f64 = sizeof(R) == 8;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;          // thread of the block (kGradTeam threads share one quartet)

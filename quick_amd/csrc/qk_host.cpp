@@ -1285,7 +1285,11 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 }
             } else {
                 const size_t lds = grad_lds_bytes(kGradBlobs[k], fp32);
-                const int grid = (int)std::min<long long>((long long)T.size(), 256LL * 16);
+                // at most 16 blocks per CU (with one-wavefront blocks: four
+                // wavefronts per SIMD), 20 for the FP32 form (five), except for
+                // dppp, measured slower at five (13.0 -> 16.0 ms on (H2O)32)
+                const bool dppp = kClassL[k][0] == 2 && kClassL[k][1] == 1 && kClassL[k][2] == 1 && kClassL[k][3] == 1;
+                const int grid = (int)std::min<long long>((long long)T.size(), 256LL * (fp32 && !dppp ? 20 : 16));
                 if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, s)) {
                     set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
                     return -1;
