@@ -30,9 +30,10 @@ struct TeamLayout {
     static constexpr int LT = LA + LB + LC + LD;
     static constexpr int kV = TT::kSlots > NAB * TT::kNF + NOUT ? TT::kSlots : NAB * TT::kNF + NOUT;
     static constexpr int kDbl = NCD + NAB + NB * ND + NB * NC + NA * ND + NA * NC;
-    // V | acc | D blocks | products of AB, CD powers by power code (64 + 64) | F_m
-    static constexpr int oAcc = kV, oD = oAcc + NACC, oPw = oD + kDbl, oF = oPw + 128;
-    static constexpr int total = oF + 16;
+    // V | acc | D blocks | products of AB, CD powers by power code (64 + 64) | F_m |
+    // the primitive quartet's VRR factors (16)
+    static constexpr int oAcc = kV, oD = oAcc + NACC, oPw = oD + kDbl, oF = oPw + 128, oGeo = oF + 16;
+    static constexpr int total = oGeo + 16;
 };
 
 template <int LA, int LB, int LC, int LD, class TT, typename R = double>
@@ -90,6 +91,10 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
     // bra HRR term; pwc[64 + c] the same in C-D for the ket HRR
     R* pwc = lds + Lo::oPw;
     R* Fl = lds + Lo::oF;
+    // VRR factors indexed by side (0 bra, 1 ket) and direction: PA|QC (0-5),
+    // WP|WQ (6-11), 1/2zeta|1/2eta (12-13), rho/zeta|rho/eta (14-15); read by
+    // index, the per-node choice needs no branches
+    R* geo = lds + Lo::oGeo;
     // QUICK's fit coefficients (the FP32 path reads its Boys table from memory)
     constexpr bool f64 = sizeof(R) == 8;
     __shared__ double s_boys[f64 ? kBoysFitRegions * kBoysFitTerms : 1];
@@ -161,6 +166,13 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                     const R PAx = Px - Ax, PAy = Py - Ay, PAz = Pz - Az, WPx = Wx - Px, WPy = Wy - Py, WPz = Wz - Pz;
                     const R QCx = Qx - Cx, QCy = Qy - Cy, QCz = Qz - Cz, WQx = Wx - Qx, WQy = Wy - Qy, WQz = Wz - Qz;
                     const R oo2ze = R(0.5) * inv;
+                    if (lane == 0) {
+                        geo[0] = PAx; geo[1] = PAy; geo[2] = PAz;
+                        geo[3] = QCx; geo[4] = QCy; geo[5] = QCz;
+                        geo[6] = WPx; geo[7] = WPy; geo[8] = WPz;
+                        geo[9] = WQx; geo[10] = WQy; geo[11] = WQz;
+                        geo[12] = oo2z; geo[13] = oo2e; geo[14] = roz; geo[15] = roe;
+                    }
                     __syncthreads();
                     for (int k = lev[0] + lane; k < lev[1]; k += 64) {
                         const uint4 nd = nodes[k];
@@ -183,13 +195,12 @@ __global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassW
                             if (kb + u * 64 >= k1) break;
                             const uint4 nd = ndb[u];
                             const unsigned fl = nd.w;
-                            const bool bra = (fl & 3u) == 1u;
-                            const int dir = (fl >> 2) & 3;
+                            const int side = (fl & 3u) == 1u ? 0 : 1;
+                            const int gi = side * 3 + int((fl >> 2) & 3u);
                             const R n1 = R((fl >> 4) & 15u), ne = R((fl >> 8) & 15u);
-                            const R pa = dir == 0 ? (bra ? PAx : QCx) : (dir == 1 ? (bra ? PAy : QCy) : (bra ? PAz : QCz));
-                            const R wp = dir == 0 ? (bra ? WPx : WQx) : (dir == 1 ? (bra ? WPy : WQy) : (bra ? WPz : WQz));
-                            const R c3 = n1 * (bra ? oo2z : oo2e);
-                            const R c4 = bra ? roz : roe;
+                            const R pa = geo[gi], wp = geo[6 + gi];
+                            const R c3 = n1 * geo[12 + side];
+                            const R c4 = geo[14 + side];
                             const R v = pa * V[nd.x >> 16] + wp * V[nd.y & 0xffffu] +
                                         c3 * (V[nd.y >> 16] - c4 * V[nd.z & 0xffffu]) + ne * oo2ze * V[nd.z >> 16];
                             V[nd.x & 0xffffu] = v;
