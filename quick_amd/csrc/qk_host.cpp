@@ -795,6 +795,11 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         const int nrow = ncart_of(kClassL[k][0]) + ncart_of(kClassL[k][1]);
         const size_t rows_lds = sizeof(unsigned long long) * (size_t)nrow * n;
         const bool use_runs = !fock_class_is_team(k) && rows_lds <= kRowsLdsMax;
+        // dynamic LDS of the thread-per-quartet kernels: the exchange rows and
+        // one bra slab (run mode), or four bra slabs (7 x 64 values each)
+        const size_t slab64 = 7 * 64 * sizeof(double), slab32 = 7 * 64 * sizeof(float);
+        const size_t lds64 = use_runs ? rows_lds + slab64 : 4 * slab64;
+        const size_t lds32 = use_runs ? rows_lds + slab32 : 4 * slab32;
         if (!CH.tiles.empty()) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
             a.counters = c->d_counters.p + 2 * k;
@@ -812,15 +817,14 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                     launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
                                 (long long)CH.segruns.size(), CH.d_segs.p, CH.d_list.p, CH.d_counts.p};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), rows_lds,
-                                      s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), lds64, s);
                 } else if (use_runs) {
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_runs.p, (long long)CH.runs.size()};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), rows_lds, s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), lds64, s);
                 } else {
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
                     const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
-                    launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), 0, s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), lds64, s);
                 }
             }
             HIPCHK(hipGetLastError());
@@ -836,7 +840,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
             ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
             int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
-            size_t lds = 0;
+            size_t lds = lds32;
             if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 16);
             if (use_runs && CH.compact32) {
                 launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
@@ -846,12 +850,12 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 w.list = CH.d_list32.p;
                 w.counts = CH.d_counts32.p;
                 grid = (int)std::min<long long>(w.nruns, cap);
-                lds = rows_lds;
+                lds = lds32;
             } else if (use_runs) {
                 w.runs = CH.d_runs32.p;
                 w.nruns = (long long)CH.runs32.size();
                 grid = (int)std::min<long long>(w.nruns, cap);
-                lds = rows_lds;
+                lds = lds32;
             }
             if (fock_class_is_team(k)) launch_team_f32_class(k, a, w, grid, s);
             else launch_fock_f32_class(k, a, w, grid, lds, s);

@@ -265,17 +265,17 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     __shared__ double s_boys[kBoysFitRegions * kBoysFitTerms];
     for (int k = threadIdx.x; k < kBoysFitRegions * kBoysFitTerms; k += blockDim.x) s_boys[k] = kBoysFit[k];
     __syncthreads();
-    // the current bra's primitive pairs (at most 64), one slab per wavefront
-    __shared__ double s_bra[4][7 * 64];
-    double* bra = s_bra[threadIdx.x >> 6];
-
     // one tile per wavefront and iteration: a bra pair shared by all lanes and
     // one ket pair per lane; every lane reaches the reduction below
-    // exchange rows of the current bra in LDS (run mode); dynamic shared memory
+    // dynamic LDS: in run mode the exchange rows of the run's bra, then that
+    // bra's primitive pairs (at most 64; one slab for the block, staged once per
+    // run); otherwise one slab per wavefront, staged per tile
     extern __shared__ unsigned long long s_xrows[];
     const bool runs = w.runs != nullptr;
     unsigned long long* const xr = runs ? s_xrows : nullptr;
     const int wib = threadIdx.x >> 6;
+    double* const bra = runs ? reinterpret_cast<double*>(s_xrows + (NA + NB) * n)
+                             : reinterpret_cast<double*>(s_xrows) + wib * (7 * 64);
     const bool compact = runs && w.segs != nullptr;
     int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
@@ -289,6 +289,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         rp = compact ? w.segs[rg.x].x : w.tiles[rg.x].p;
         c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
+        if (wib == 0) stage_bra(a.pp, a.pr.prim0[rp], a.pr.pstride[rp], a.pr.nprim[rp], lane, bra);
         __syncthreads();
     } else {
         t0 = wave0; t1 = w.ntiles; tstep = nwaves;
@@ -342,7 +343,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0;
 
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra(a.pp, pb0, pbs, nbp, lane, bra);
+        if (!runs) stage_bra(a.pp, pb0, pbs, nbp, lane, bra);   // run mode: p is the run's bra, staged
 
         if (active) {
             ++nsurv;
@@ -604,8 +605,6 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
     const float plim = float(a.prim_limit);
-    __shared__ float s_braf[4][7 * 64];
-    float* bra = s_braf[threadIdx.x >> 6];
     constexpr bool kBoysLds = NA + NB <= 4;   // two or four exchange rows
     __shared__ float s_btab[kBoysLds ? kBoysTabRows * kBoysLdsCols : 1];
     if constexpr (kBoysLds) {
@@ -613,11 +612,15 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
         __syncthreads();
     }
 
-    // exchange rows of the current bra in LDS (run mode); dynamic shared memory
+    // dynamic LDS: in run mode the exchange rows of the run's bra, then that
+    // bra's primitive pairs (one slab for the block, staged once per run);
+    // otherwise one slab per wavefront, staged per tile
     extern __shared__ unsigned long long s_xrows[];
     const bool runs = w.runs != nullptr;
     unsigned long long* const xr = runs ? s_xrows : nullptr;
     const int wib = threadIdx.x >> 6;
+    float* const bra = runs ? reinterpret_cast<float*>(s_xrows + (NA + NB) * n)
+                            : reinterpret_cast<float*>(s_xrows) + wib * (7 * 64);
     const bool compact = runs && w.segs != nullptr;
     int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
@@ -631,6 +634,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
         rp = compact ? w.segs[rg.x].x : w.tiles[rg.x].p;
         c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
+        if (wib == 0) stage_bra(a.ppf, a.pr.prim0[rp], a.pr.pstride[rp], a.pr.nprim[rp], lane, bra);
         __syncthreads();
     } else {
         t0 = wave0; t1 = w.ntiles; tstep = nwaves;
@@ -682,7 +686,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) jab[k] = 0.0f;
         const int pb0 = a.pr.prim0[p], pbs = a.pr.pstride[p], nbp = a.pr.nprim[p];
-        stage_bra(a.ppf, pb0, pbs, nbp, lane, bra);
+        if (!runs) stage_bra(a.ppf, pb0, pbs, nbp, lane, bra);   // run mode: p is the run's bra, staged
 
         if (active) {
             ++nsurv;
