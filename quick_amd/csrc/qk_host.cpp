@@ -160,11 +160,31 @@ static void tile_runs(const std::vector<Tile>& T, std::vector<int2>& runs, int c
     }
 }
 
-static void seg_runs(const std::vector<int4>& S, std::vector<int2>& runs) {
+// the same over compaction segments (push_segment keeps each within cap)
+static void seg_runs(const std::vector<int4>& S, std::vector<int2>& runs, int cap) {
     runs.clear();
-    for (size_t k = 0; k < S.size(); ++k)
-        if (runs.empty() || S[runs.back().x].x != S[k].x) runs.push_back({(int)k, (int)k + 1});
-        else runs.back().y = (int)k + 1;
+    int cands = 0;
+    for (size_t k = 0; k < S.size(); ++k) {
+        const int len = S[k].z - S[k].y;
+        if (runs.empty() || S[runs.back().x].x != S[k].x || cands + len > cap) {
+            runs.push_back({(int)k, (int)k + 1});
+            cands = 0;
+        } else {
+            runs.back().y = (int)k + 1;
+        }
+        cands += len;
+    }
+}
+
+// the candidate kets [q0, q1) of bra pair p as segments of at most cap kets,
+// their compacted lists from offset off on
+static void push_segment(std::vector<int4>& S, int p, int q0, int q1, int cap, long long& off) {
+    for (int q = q0; q < q1;) {
+        const int e = q1 - q > cap ? q + cap : q1;
+        S.push_back({p, q, e, (int)off});
+        off += e - q;
+        q = e;
+    }
 }
 
 // Boys values F_m(k / kBoysTabPerUnit) for the FP32 kernels, from the series
@@ -585,6 +605,16 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         ClassHost& CH = c->classes[k];
         long long total = 0, total32 = 0, seg_off = 0, seg_off32 = 0;
+        // Runs (one bra pair per block) of at most 2048 kets where one strong
+        // bra pair's kets would otherwise hold a single block for much of the
+        // kernel: (ss|ss), and the FP64 list of a mixed build (the strongest
+        // pairs: few bras, long ket ranges).  Measured on (H2O)32: (ss|ss) FP64
+        // 3.40 -> 2.49 ms, FP32 1.41 -> 1.22 ms; mixed FP64 lists 0.4 ms less in
+        // total (psps 0.61 -> 0.50 ms).  Elsewhere the cap was slower (psss FP64
+        // 4.40 -> 5.15 ms): every run zeroes and adds its exchange rows.
+        const bool cap_ss = kClassL[k][0] + kClassL[k][1] == 0;
+        const int run_cap = cap_ss || mixed ? 2048 : INT_MAX;
+        const int run_cap32 = cap_ss ? 2048 : INT_MAX;
         for (size_t gb = 0; gb < groups.size(); ++gb) {
             if (groups[gb].cls != cb) continue;
             const Group& G1 = groups[gb];
@@ -621,30 +651,20 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
                     for (int q = lo64; q < lo; q += chunk)
                         CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
                     if (!fock_class_is_team(k)) {
-                        if (lo64 > q_begin) {
-                            CH.segs.push_back({G1.start + i, G2.start + q_begin, G2.start + lo64, (int)seg_off});
-                            seg_off += lo64 - q_begin;
-                        }
-                        if (lo > lo64) {
-                            CH.segs32.push_back({G1.start + i, G2.start + lo64, G2.start + lo, (int)seg_off32});
-                            seg_off32 += lo - lo64;
-                        }
+                        if (lo64 > q_begin)
+                            push_segment(CH.segs, G1.start + i, G2.start + q_begin, G2.start + lo64, run_cap, seg_off);
+                        if (lo > lo64)
+                            push_segment(CH.segs32, G1.start + i, G2.start + lo64, G2.start + lo, run_cap32, seg_off32);
                     }
                     if (lo > q_begin) total += lo - q_begin;
                     if (lo > lo64) total32 += lo - lo64;
                 }
             }
         }
-        // (ss|ss) runs hold at most 2048 kets: a strong bra pair's thousands of
-        // kets then spread over several blocks instead of one block setting the
-        // kernel's length (measured on (H2O)32: FP64 3.40 -> 2.49 ms, FP32 1.41
-        // -> 1.22 ms); the classes with more exchange rows ran slower with the
-        // same cap (psss FP64 4.40 -> 5.15 ms: each run zeroes and adds its rows)
-        const int run_cap = kClassL[k][0] + kClassL[k][1] == 0 ? 2048 : INT_MAX;
         tile_runs(CH.tiles, CH.runs, run_cap);
-        tile_runs(CH.tiles32, CH.runs32, run_cap);
-        seg_runs(CH.segs, CH.segruns);
-        seg_runs(CH.segs32, CH.segruns32);
+        tile_runs(CH.tiles32, CH.runs32, run_cap32);
+        seg_runs(CH.segs, CH.segruns, run_cap);
+        seg_runs(CH.segs32, CH.segruns32, run_cap32);
         if (seg_off > INT32_MAX || seg_off32 > INT32_MAX) {
             set_err("class %d: more than 2^31 candidate quartets", k);
             return -1;
