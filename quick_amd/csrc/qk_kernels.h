@@ -357,8 +357,8 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
             for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
 
             // kets outer (vector loads; the next ket primitive is fetched while the
-            // current one runs), bra primitives inner, read from the lanes that hold
-            // them (v_readlane: no memory access in the inner loop)
+            // current one runs), bra primitives inner, read from the bra slab in LDS
+            // (the same address on every lane: broadcast reads)
             const int pk0 = a.pr.prim0[q], pks = a.pr.pstride[q], nkp = a.pr.nprim[q];
             {
                 const int ni = nbp;   // <= 64 primitive pairs per Qshell pair (checked on the host)
