@@ -185,3 +185,8 @@ def test_grad_mixed_full_size(system):
     print(f"{system}: fp32 share {share:.3f} max|dgrad| {d:.2e} (max|grad| {np.abs(g64).max():.3f})", flush=True)
     assert share > 0.5
     assert d <= TOL_G_MIXED
+    # threshold 0: all FP64, but most light classes then run from their tiles
+    # where the FP64 gradient runs them density-compacted: bit for bit equal
+    g0 = eng.grad(D, make_opts(precision=QK_MIXED, fp32_threshold=0.0))
+    assert _fp32_share(eng)[0] == 0.0
+    assert np.array_equal(g0, g64)

@@ -85,3 +85,11 @@ def test_mixed_water32_threshold_sweep():
         if thr <= 1e-5:
             assert dE <= TOL_E_MIXED, (thr, dE)
             assert share > 0.0
+    # threshold 0: every quartet in FP64, but through the mixed build's work
+    # lists (its FP64 runs are cut at 2048 kets and their compaction segments
+    # split to match, where the FP64 build keeps one run per bra pair): the
+    # same quartets with the same fixed-point contributions, so bit for bit the
+    # FP64 build
+    G0 = eng.fock(D, make_opts(precision=QK_MIXED, fp32_threshold=0.0))
+    assert eng.stats().quartets_fp32 == 0
+    assert np.array_equal(G0, G64)
