@@ -42,19 +42,20 @@ __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const
     const int lane = threadIdx.x & 63;
     if (s >= nsegs) return;   // the whole wavefront
     const int4 sg = segs[s];
-    const int qa = a.qa[sg.x], qb = a.qb[sg.x];
-    const int sA = a.shell[qa], sB = a.shell[qb];
-    const int atA = a.atom[qa], atB = a.atom[qb];
+    const int shp = a.psh[sg.x], atp = a.pat[sg.x];
+    const int sA = shp & 0xffff, sB = shp >> 16;
+    const int atA = atp & 0xffff, atB = atp >> 16;
     const double yp = a.Y[sg.x];
     int cnt = 0;
     for (int q0 = sg.y; q0 < sg.z; q0 += 64) {
         const int q = q0 + lane;
         bool keep = false;
         if (q < sg.z) {
-            const int qc = a.qa[q], qd = a.qb[q];
-            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, a.shell[qc], a.shell[qd]);
+            const int shq = a.psh[q];
+            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, shq & 0xffff, shq >> 16);
             if (a.grad) {
-                const int atC = a.atom[qc], atD = a.atom[qd];
+                const int atq = a.pat[q];
+                const int atC = atq & 0xffff, atD = atq >> 16;
                 const double ct = a.Y[q] * yp;
                 keep = !(atA == atB && atA == atC && atA == atD) && ct > a.tau && ct * dn > a.tau;
             } else {

@@ -272,6 +272,7 @@ struct qk_ctx {
     DBuf<double> d_xyz, d_cons;
     DBuf<int> dq_atom, dq_f0, dq_shell, dq_qf;
     DBuf<int> dp_qa, dp_qb, dp_prim0, dp_nprim;
+    DBuf<int> dp_sh, dp_at;   // per pair: shells and atoms of its two Qshells, 16 bits each (< 2^15)
     DBuf<double> dp_Y;
     DBuf<double> dpp_zeta, dpp_oo2z, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut;
     DBuf<int> d_sh_f0, d_sh_nf;
@@ -569,12 +570,15 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
             groups.push_back({c->ap_cls[e], k, 0});
         groups.back().count++;
     }
-    std::vector<int> qa(npair), qb(npair), prim0(npair), pstride(npair), nprim(npair);
+    std::vector<int> qa(npair), qb(npair), prim0(npair), pstride(npair), nprim(npair), psh(npair), pat(npair);
     std::vector<double> Yp(npair);
+    if (c->nshell > 0x7fff || c->natom > 0x7fff) { set_err("more than 32767 shells or atoms"); return -1; }
     for (int k = 0; k < npair; ++k) {
         const int e = sel[k];
         qa[k] = c->ap_qa[e]; qb[k] = c->ap_qb[e]; nprim[k] = c->ap_nprim[e];
         Yp[k] = c->ap_Y[e];
+        psh[k] = c->q_shell[qa[k]] | (c->q_shell[qb[k]] << 16);
+        pat[k] = c->q_atom[qa[k]] | (c->q_atom[qb[k]] << 16);
     }
     // primitive-major copy per group: entry (pair k, primitive j) at
     // base(group) + j * count(group) + (k - start(group))
@@ -677,7 +681,8 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_pstride.upload(pstride) ||
         c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dk_zeta.upload(kz) || c->dk_oo2z.upload(ko) ||
         c->dk_Px.upload(kx) || c->dk_Py.upload(ky) || c->dk_Pz.upload(kzz) || c->dk_K.upload(kK) ||
-        c->dk_cut.upload(kc) || c->dk_ea.upload(kea) || c->dk_eb.upload(keb))
+        c->dk_cut.upload(kc) || c->dk_ea.upload(kea) || c->dk_eb.upload(keb) || c->dp_sh.upload(psh) ||
+        c->dp_at.upload(pat))
         return -1;
     c->grp_cls.clear(); c->grp_start.clear(); c->grp_count.clear();
     for (const Group& g : groups) {
@@ -763,12 +768,10 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
 }
 
 // inputs of the density compaction: the Fock test (grad 0) or the gradient's
-static CompactArgs compact_args(const FockArgs& a, int grad, double tau) {
+static CompactArgs compact_args(const qk_ctx* c, const FockArgs& a, int grad, double tau) {
     CompactArgs ca;
-    ca.qa = a.pr.qa;
-    ca.qb = a.pr.qb;
-    ca.shell = a.q.shell;
-    ca.atom = a.q.atom;
+    ca.psh = c->dp_sh.p;
+    ca.pat = c->dp_at.p;
     ca.Y = a.pr.Y;
     ca.cutmat = a.cutmat;
     ca.nshell = a.nshell;
@@ -809,7 +812,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);
-    const CompactArgs ca = compact_args(a, 0, a.tau);
+    const CompactArgs ca = compact_args(c, a, 0, a.tau);
     if (fold_timing(c)) return -1;
     // heaviest classes first, round-robin over the streams
     std::vector<int> order(kNumQuartetClasses);
@@ -1281,10 +1284,8 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     a.Dpf = c->d_Dpf.p;
     a.boys_tab = c->d_boys_tab.p;
     CompactArgs ca;   // the gradient's screen (one-centre skip, Y Y > cutoff, Y Y DNmax > cutoff)
-    ca.qa = a.pr.qa;
-    ca.qb = a.pr.qb;
-    ca.shell = a.q.shell;
-    ca.atom = a.q.atom;
+    ca.psh = c->dp_sh.p;
+    ca.pat = c->dp_at.p;
     ca.Y = a.pr.Y;
     ca.cutmat = a.cutmat;
     ca.nshell = ns;

@@ -87,13 +87,11 @@ struct ClassWork {
     const Tile* steps;
 };
 
-// Inputs of the per-build density compaction (compact_kernel): the pair
-// lists, shells and atoms of the Qshells, Schwarz bounds and density screen.
+// Inputs of the per-build density compaction (compact_kernel): per pair its
+// two shells and two atoms (packed), Schwarz bounds and the density screen.
 struct CompactArgs {
-    const int* qa;
-    const int* qb;
-    const int* shell;
-    const int* atom;
+    const int* psh;   // per pair: parent shells of its Qshells (bits 0-15, 16-31)
+    const int* pat;   // per pair: atoms of its Qshells (bits 0-15, 16-31)
     const double* Y;
     const double* cutmat;
     int nshell;
