@@ -879,7 +879,8 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
             int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
             size_t lds = lds32;
-            if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 16);
+            // one 64-lane block per tile; up to 20 resident per CU (five wavefronts per SIMD)
+            if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 20);
             if (use_runs && CH.compact32) {
                 launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
                 w.runs = CH.d_segruns32.p;

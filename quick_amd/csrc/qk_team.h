@@ -69,8 +69,10 @@ struct TeamIn<float> {
     }
 };
 
+// FP32: five wavefronts per SIMD (measured on (H2O)32: dpdp 1.31 -> 1.20 ms,
+// ddds 0.86 -> 0.80 ms; asking the FP64 form for three made it slower)
 template <int LA, int LB, int LC, int LD, class TT, typename R>
-__global__ void __launch_bounds__(64) team_kernel(const FockArgs a, const ClassWork w) {
+__global__ void __launch_bounds__(64, (sizeof(R) == 4 ? 5 : 1)) team_kernel(const FockArgs a, const ClassWork w) {
     using Lo = TeamLayout<LA, LB, LC, LD, TT>;
     constexpr int NA = Lo::NA, NB = Lo::NB, NC = Lo::NC, ND = Lo::ND;
     constexpr int NAB = Lo::NAB, NCD = Lo::NCD, NOUT = Lo::NOUT, NACC = Lo::NACC, LT = Lo::LT;
