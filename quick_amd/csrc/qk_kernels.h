@@ -587,7 +587,12 @@ __device__ __forceinline__ void fx_add_f(unsigned long long* X, long long idx, f
 template <int LA, int LB, int LC, int LD>
 struct OccupancyF {
     static constexpr int nacc = nrange(LA, LA + LB) * nrange(LC, LC + LD);
-    static constexpr int value = nacc <= 36 ? 4 : 2;
+    // six or more exchange rows already hold a run-mode block to three per CU
+    // at 800 functions (more functions: fewer): asking for four wavefronts per
+    // SIMD only made such kernels spill (measured: ppps 1.34 -> 1.27 ms, dsds
+    // 0.44 -> 0.42 ms with three)
+    static constexpr int rows = ncart(LA) + ncart(LB);
+    static constexpr int value = nacc <= 36 ? (rows >= 6 ? 3 : 4) : 2;
 };
 
 template <int LA, int LB, int LC, int LD>
