@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdarg>
@@ -13,6 +14,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/quick_mi355x.h"
@@ -174,6 +176,23 @@ static void seg_runs(const std::vector<int4>& S, std::vector<int2>& runs, int ca
         }
         cands += len;
     }
+}
+
+// f(k) for every quartet class on up to 16 host threads: the classes' work
+// lists are independent (f touches only class k's vectors and makes no HIP
+// call); classes are taken in index order, the large s/p classes first
+template <class F>
+static void for_each_class(F&& f) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)kNumQuartetClasses);
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k < kNumQuartetClasses; k = next.fetch_add(1)) f(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& th : pool) th.join();
 }
 
 // the candidate kets [q0, q1) of bra pair p as segments of at most cap kets,
@@ -604,7 +623,8 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     // the prefix of kets passing Y_ij*Y_kl > tau cut into runs of <= 64
     c->classes.clear();
     c->classes.resize(kNumQuartetClasses);
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
+    std::vector<char> too_big(kNumQuartetClasses, 0);
+    for_each_class([&](int k) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         ClassHost& CH = c->classes[k];
@@ -669,13 +689,15 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         tile_runs(CH.tiles32, CH.runs32, run_cap32);
         seg_runs(CH.segs, CH.segruns, run_cap);
         seg_runs(CH.segs32, CH.segruns32, run_cap32);
-        if (seg_off > INT32_MAX || seg_off32 > INT32_MAX) {
+        too_big[k] = seg_off > INT32_MAX || seg_off32 > INT32_MAX;
+        CH.total = total;
+        CH.total32 = total32;
+    });
+    for (int k = 0; k < kNumQuartetClasses; ++k)
+        if (too_big[k]) {
             set_err("class %d: more than 2^31 candidate quartets", k);
             return -1;
         }
-        CH.total = total;
-        CH.total32 = total32;
-    }
 
     (void)hipSetDevice(c->device);
     if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_pstride.upload(pstride) ||
@@ -1139,11 +1161,15 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
         (!mixed || c->grad_prep_thr == o->fp32_threshold))
         return 0;
     (void)hipSetDevice(c->device);
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
+    for (int k = 0; k < kNumQuartetClasses; ++k)
         if (!c->d_gtab[k].p) {
             std::vector<int> blob(kGradBlobs[k], kGradBlobs[k] + kGradBlobLen[k]);
             if (c->d_gtab[k].upload(blob)) return -1;
         }
+    // the work lists of every class on host threads, then the uploads here
+    std::vector<long long> goff(2 * kNumQuartetClasses, 0);
+    std::vector<char> too_big(kNumQuartetClasses, 0);
+    for_each_class([&](int k) {
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
@@ -1206,16 +1232,24 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
                 }
             }
         }
-        if (off[0] > INT32_MAX || off[1] > INT32_MAX || T.size() > INT32_MAX || T32.size() > INT32_MAX) {
+        too_big[k] = off[0] > INT32_MAX || off[1] > INT32_MAX || T.size() > INT32_MAX || T32.size() > INT32_MAX;
+        goff[2 * k] = off[0];
+        goff[2 * k + 1] = off[1];
+    });
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        if (too_big[k]) {
             set_err("gradient class %d: more than 2^31 candidate quartets", k);
             return -1;
         }
+        const std::vector<Tile>& T = c->gtiles[k];
+        const std::vector<Tile>& T32 = c->gtiles32[k];
         if (c->d_gtiles[k].upload(T) || c->d_gtiles32[k].upload(T32)) return -1;
+        const bool light = grad_class_is_light(k);
         for (int f = 0; f < 2; ++f) {
             const size_t nslot = light ? (f ? T32.size() : T.size()) : 0;
             if (c->d_gsegs[f][k].upload(c->gsegs[f][k]) || c->d_gslot0[f][k].upload(c->gslot0[f][k]) ||
                 c->d_gsteps[f][k].alloc(std::max<size_t>(nslot, 1)) ||
-                c->d_glist[f][k].alloc((size_t)std::max(off[f], 1LL)) ||
+                c->d_glist[f][k].alloc((size_t)std::max(goff[2 * k + f], 1LL)) ||
                 c->d_gcounts[f][k].alloc(std::max<size_t>(c->gsegs[f][k].size(), 1)))
                 return -1;
         }
