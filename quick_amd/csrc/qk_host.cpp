@@ -79,17 +79,20 @@ extern "C" const char* qk_last_error(void) { return g_err.c_str(); }
 template <typename T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;     // elements in use
+    size_t cap = 0;   // elements allocated: a smaller count reuses the buffer
+                      // (work lists are rebuilt per option set; hipFree and
+                      // hipMalloc of hundreds of buffers each time cost more)
     int alloc(size_t count) {
-        if (count == n && p) return 0;
+        if (count == 0) { release(); return 0; }
+        if (p && count <= cap) { n = count; return 0; }
         release();
-        if (count == 0) return 0;
         if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
             p = nullptr;
             set_err("hipMalloc of %zu bytes failed", count * sizeof(T));
             return -1;
         }
-        n = count;
+        n = cap = count;
         return 0;
     }
     int upload(const std::vector<T>& v) {
@@ -103,7 +106,7 @@ struct DBuf {
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
     ~DBuf() { release(); }
 };
