@@ -146,6 +146,7 @@ def main():
     sptr = stream.cuda_stream
 
     prep_s = {}
+    repro = {}
 
     def timed(opts, label):
         """warmup + `steps` timed builds with `opts` (no per-kernel events inside
@@ -174,7 +175,10 @@ def main():
         for _ in range(max(1, min(args.steps, 5))):
             eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
         torch.cuda.synchronize()
-        log(f"{label}: {args.steps} builds {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build")
+        # the fixed-point accumulation makes every build bit-identical
+        repro[label] = bool(np.array_equal(G, Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")))
+        log(f"{label}: {args.steps} builds {ms:.3f} ms/build (event), wall {wall * 1e3 / args.steps:.3f} ms/build"
+            f"{'' if repro[label] else ' (NOT bit-identical across builds)'}")
         # kernel rows: (kernel, tag, quartets, prim quartets, flops, ms, precision)
         rows = [(f"{kernel_of(c[0])}<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp64") for c in eng.class_stats()]
         rows += [(f"{kernel_of(c[0])}_f32<{c[0]}>", c[0], c[1], c[2], c[3], c[4], "fp32")
@@ -344,10 +348,18 @@ def main():
         # per geometry: Schwarz bounds on the device, then the host's work lists
         # per option set (outside the timed builds)
         "setup_s": {"schwarz": setup_s, "prepare": prep_s},
+        "bit_identical_builds": repro,
         "mixed_sweep": sweep,
         "eri_quartets_per_s": st.quartets * value,
         "achieved_tflops_total": (total64 + total32) / (ms * 1e-3) / 1e12,
         "vector_roofline_time_frac": t_peak_ms / ms,
+        # the three longest class kernels of the headline build (the first is
+        # the roofline kernel above; near-equal ones can swap places run to run)
+        "roofline_top3": [{"kernel": r[0], "avg_launch_ms": r[5],
+                           "achieved_tflops": r[4] / (r[5] * 1e-3) / 1e12,
+                           "frac": r[4] / (r[5] * 1e-3) / 1e12 / (PEAK_FP64_TFLOPS if r[6] == "fp64"
+                                                                  else PEAK_FP32_TFLOPS)}
+                          for r in sorted(rows, key=lambda r: -r[5])[:3]],
         "roofline": {"bound": "mfma", "pipe": "VALU " + kprec, "kernel": kname, "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": kms},
