@@ -66,25 +66,25 @@ struct Tile {
 };
 
 struct ClassWork {
-    const Tile* tiles;
-    long long ntiles;
+    const Tile* tiles = nullptr;
+    long long ntiles = 0;
     // optional: runs of tiles sharing one bra pair (first tile, end tile); when
     // set, a block takes one run at a time and sums the exchange contributions
     // to the bra's rows of X in LDS (n columns per bra function) before adding
     // them to X once per run
-    const int2* runs;
-    long long nruns;
+    const int2* runs = nullptr;
+    long long nruns = 0;
     // optional (with runs): the density-screened quartets compacted per
     // segment (one bra pair, one ket group): segs[s] = {bra pair, first ket,
     // end ket, offset}, the surviving kets of segment s at list[offset ..
     // offset + counts[s]); runs then index segments
-    const int4* segs;
-    const int* list;
-    const int* counts;
+    const int4* segs = nullptr;
+    const int* list = nullptr;
+    const int* counts = nullptr;
     // optional (without runs): the steps written by compact_kernel, one per
     // tile and in tile order: tiles whose kets are list[q0 .. q0 + len)
     // (len 0: nothing survived there)
-    const Tile* steps;
+    const Tile* steps = nullptr;
 };
 
 // Inputs of the per-build density compaction (compact_kernel): per pair its
