@@ -72,10 +72,10 @@ __device__ __forceinline__ void gx_add(unsigned long long* G, long long idx, dou
 // fits 168 VGPRs (76 bytes of spills) instead of 205 at two, the FP32 form 96
 // (no spills) instead of 100 at four.  Measured on (H2O)32: FP64 gradient
 // 174.5 -> 172.4 ms (dpdp 14.3 -> 11.0 ms); the mixed gradient's FP32 classes
-// gain 0.3-0.9 ms each at five (dpdp 6.7 -> 5.7 ms), except dppp, which the
-// host keeps at four resident wavefronts (16.0 ms at five against 13.0).
-template <int kGradTeam, typename R>
-__global__ void __launch_bounds__(kGradTeam, (sizeof(R) == 4 ? 5 : 3)) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
+// gain 0.3-0.9 ms each at five (dpdp 6.7 -> 5.7 ms), except dppp (16.0 ms at
+// five), which runs a form built for four (kMinWaves 4).
+template <int kGradTeam, typename R, int kMinWaves = (sizeof(R) == 4 ? 5 : 3)>
+__global__ void __launch_bounds__(kGradTeam, kMinWaves) grad_kernel(const GradArgs a, const ClassWork w, const int* __restrict__ tab) {
     constexpr bool f64 = sizeof(R) == 8;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;          // thread of the block (kGradTeam threads share one quartet)
@@ -372,19 +372,22 @@ size_t grad_lds_bytes(const int* hdr, bool fp32) {
     return sizeof(double) * 36 + (fp32 ? sizeof(float) : sizeof(double)) * (size_t)(RV + nS + hdr[kGhNOut] + 128 + 16 + 16);
 }
 
-template <int TEAM, typename R>
+template <int TEAM, typename R, int W = (sizeof(R) == 4 ? 5 : 3)>
 static int launch_team(const GradArgs& a, const ClassWork& w, const int* dtab, size_t lds, int grid, hipStream_t s) {
     if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM, R>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM, R, W>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL((grad_kernel<TEAM, R>), dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
+    hipLaunchKernelGGL((grad_kernel<TEAM, R, W>), dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // hdr: the class's table header (host copy); dtab: the same table on the device
+// four_waves (FP32, one-wavefront teams): the form built for four wavefronts
+// per SIMD (100 VGPRs) instead of five (96), for a class that runs slower at five
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
-                      bool fp32, hipStream_t s) {
+                      bool fp32, bool four_waves, hipStream_t s) {
+    if (fp32 && four_waves && grad_team_of(hdr) == 64) return launch_team<64, float, 4>(a, w, dtab, lds, grid, s);
     if (fp32)
         return grad_team_of(hdr) == 256 ? launch_team<256, float>(a, w, dtab, lds, grid, s)
                                         : launch_team<64, float>(a, w, dtab, lds, grid, s);

@@ -34,7 +34,7 @@ void launch_team_f32_class(int k, const FockArgs& a, const ClassWork& w, int gri
 void launch_scale_density(const double* D, const double* cons, double* Dp, float* Dpf, int n, hipStream_t s);
 size_t grad_lds_bytes(const int* hdr, bool fp32);
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
-                      bool fp32, hipStream_t s);
+                      bool fp32, bool four_waves, hipStream_t s);
 void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
 bool grad_class_is_light(int k);
 int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bool fp32, hipStream_t s);
@@ -1365,10 +1365,11 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 const size_t lds = grad_lds_bytes(kGradBlobs[k], fp32);
                 // at most 16 blocks per CU (with one-wavefront blocks: four
                 // wavefronts per SIMD), 20 for the FP32 form (five), except for
-                // dppp, measured slower at five (13.0 -> 16.0 ms on (H2O)32)
+                // dppp, measured slower at five (12.4 ms at four, 16.0 at five on
+                // (H2O)32): it runs the FP32 form built for four
                 const bool dppp = kClassL[k][0] == 2 && kClassL[k][1] == 1 && kClassL[k][2] == 1 && kClassL[k][3] == 1;
                 const int grid = (int)std::min<long long>((long long)T.size(), 256LL * (fp32 && !dppp ? 20 : 16));
-                if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, s)) {
+                if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, dppp, s)) {
                     set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
                     return -1;
                 }
