@@ -360,7 +360,7 @@ def main():
                            "frac": r[4] / (r[5] * 1e-3) / 1e12 / (PEAK_FP64_TFLOPS if r[6] == "fp64"
                                                                   else PEAK_FP32_TFLOPS)}
                           for r in sorted(rows, key=lambda r: -r[5])[:3]],
-        "roofline": {"bound": "mfma", "pipe": "VALU " + kprec, "kernel": kname, "achieved": achieved,
+        "roofline": {"bound": "vector", "pipe": "VALU " + kprec, "kernel": kname, "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": kms},
         "cpu_baseline": base,

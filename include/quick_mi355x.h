@@ -65,9 +65,12 @@ void gpu_upload_cutoff_(double* cutMatrix, double* integralCutoff, double* primL
                         double* DMCutoff);
 /* gpu.cu:1806 forward: o <- o + G(D), symmetric, written in place */
 void gpu_get2e_(double* o);
-/* gpu.cu:1393 gradient seed (3*natom) and gradient cutoff */
+/* gpu.cu:1393 gradient seed (3*natom, the caller's partial gradient, copied
+ * here) and gradient cutoff */
 void gpu_upload_grad_(double* grad, double* gradCutoff);
-/* gpu.cu:1481 backward: grad <- grad + dE_2e/dR for the uploaded density */
+/* gpu.cu:1481 backward: grad <- seed + dE_2e/dR (four-centre term) for the
+ * uploaded density.  Screening as the reference CPU path (gradient.f90:289-316,
+ * shell.f90:1941): quartet and primitive tests both against gradCutoff. */
 void gpu_grad_(double* grad);
 /* gpu.cu:1866 XC on the GPU: not part of this library; records an error */
 void gpu_getxc_(int* isg, double* sigrad2, double* Eelxc, double* aelec, double* belec,
@@ -152,6 +155,13 @@ int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 int qk_fock_build(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 int qk_fock_grad(qk_ctx* c, const double* D, const double* W, double* grad, const qk_opts* o);
 
+/* Range check of the fixed-point accumulators: waits for the device, returns 0
+ * when the last Fock build's and the last gradient's sums stayed well inside
+ * the int64 range, -5 (message in qk_last_error) when one came within a factor
+ * of two of wrapping.  The host entry points (qk_fock_host, qk_grad_host,
+ * gpu_get2e_, gpu_grad_) run this check themselves. */
+int qk_check_range(qk_ctx* c);
+
 /* Statistics of the last build: surviving contracted Qshell quartets, primitive
  * quartets evaluated (FP64 and FP32), algorithmic flops. */
 typedef struct qk_stats {
@@ -183,6 +193,11 @@ int qk_grad_class_stats(qk_ctx* c, int k, long long* quartets, long long* prim_q
 int qk_grad_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, float* ms);
 /* The same figures for class k's FP32 kernel (QK_MIXED; zero otherwise). */
 int qk_class_stats_fp32(qk_ctx* c, int k, long long* quartets, long long* prim_quartets, double* flops, float* ms);
+
+/* Largest grid (blocks of waves_per_block wavefronts) the library launches for
+ * a kernel with lane_bytes of scratch per lane: the dispatch's scratch backing,
+ * lane_bytes x 64 x waves in flight, stays under 256 MiB.  No device needed. */
+long long qk_scratch_grid_cap(long long lane_bytes, int waves_per_block);
 
 /* The FP32 kernels' Boys table: F_m(k / per_unit) for k < rows, m < cols,
  * row-major.  No device needed. */

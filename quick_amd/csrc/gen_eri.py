@@ -291,8 +291,13 @@ def write_units(outdir):
             body += [f'void launch_schwarz_{pt}(const int* pairs, int np, const int* qa, const int* qb, const int* prim0,',
                      '                        const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz,',
                      '                        const double* cons, double* ymax, double* pmax, hipStream_t s) {',
-                     f'    hipLaunchKernelGGL((schwarz_kernel<{cls[0]}, {cls[1]}>), dim3((np + 255) / 256), dim3(256), 0, s,',
-                     '                       pairs, np, qa, qb, prim0, nprim, qs, pp, xyz, cons, ymax, pmax);',
+                     f'    static const long long cap = scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&schwarz_kernel<{cls[0]}, {cls[1]}>)), 4);',
+                     '    // one thread per pair, in launches of at most cap blocks (outputs are indexed by pair)',
+                     '    for (long long off = 0; off < np; off += cap * 256) {',
+                     '        const int m = (int)std::min<long long>(np - off, cap * 256);',
+                     f'        hipLaunchKernelGGL((schwarz_kernel<{cls[0]}, {cls[1]}>), dim3((m + 255) / 256), dim3(256), 0, s,',
+                     '                           pairs + off, m, qa, qb, prim0, nprim, qs, pp, xyz, cons, ymax, pmax);',
+                     '    }',
                      '}']
         body.append('}  // namespace qk')
         fn = os.path.join(outdir, f'fock_{t}.hip')

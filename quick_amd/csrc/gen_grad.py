@@ -526,10 +526,12 @@ def write_light_units(outdir):
         body = ["// Generated by quick_amd/csrc/gen_grad.py -- do not edit.", '#include "../qk_grad_light.h"',
                 "namespace qk {",
                 f"void launch_grad_light_{t}(const GradArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{",
-                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, double>), dim3(grid), dim3(256), lds, s, a, w);",
+                f"    static const long long cap = scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, double>)), 4);",
+                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, double>), dim3((int)std::min<long long>(grid, cap)), dim3(256), lds, s, a, w);",
                 "}",
                 f"void launch_grad_light_f32_{t}(const GradArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{",
-                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, float>), dim3(grid), dim3(256), lds, s, a, w);",
+                f"    static const long long cap = scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, float>)), 4);",
+                f"    hipLaunchKernelGGL((grad_light_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, float>), dim3((int)std::min<long long>(grid, cap)), dim3(256), lds, s, a, w);",
                 "}", "}  // namespace qk"]
         fn = os.path.join(outdir, f"grad_{t}.hip")
         new = "\n".join(body) + "\n"

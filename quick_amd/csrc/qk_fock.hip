@@ -106,18 +106,22 @@ __global__ void fill_u64(unsigned long long* p, long long count) {
     if (k < count) p[k] = 0ull;
 }
 
-// G = diag(cons) (X + X^T)/2 diag(cons) in FP64, X the sum of the kXCopies copies
-__global__ void fock_finalize(const unsigned long long* __restrict__ X, const double* __restrict__ cons,
-                              double* __restrict__ G, int n) {
+// G = diag(cons) (X + X^T)/2 diag(cons) in FP64.  The sum T of the copies'
+// slot {i, j} holds X[i][j] + X[j][i] for i != j and X[i][i] on the diagonal.
+// A sum within a factor of two of the int64 range (|T| >= 2^62, i.e. |G| of
+// about 8192 and more) raises the range flag: the accumulator would wrap at
+// twice that.
+__global__ void fock_finalize(const unsigned long long* __restrict__ X, int copies, const double* __restrict__ cons,
+                              double* __restrict__ G, int n, unsigned long long* __restrict__ flag) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)n * n) return;
     const int i = int(k / n), j = int(k % n);
-    const long long nn = (long long)n * n;
+    const long long t = xtri(i, j), ts = xtri_size(n);
     unsigned long long u = 0;
-#pragma unroll
-    for (int c = 0; c < kXCopies; ++c) u += X[c * nn + (long long)i * n + j] + X[c * nn + (long long)j * n + i];
+    for (int c = 0; c < copies; ++c) u += X[c * ts + t];
     const long long s = (long long)u;
-    G[k] = cons[i] * cons[j] * ((double)s * 0x1p-49);
+    if (s >= (1LL << 62) || s <= -(1LL << 62)) atomicOr(flag, 1ull);
+    G[k] = cons[i] * cons[j] * ((double)s * (i == j ? 0x1p-48 : 0x1p-49));
 }
 
 // ---------------------------------------------------------------------------
@@ -230,8 +234,10 @@ void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
 }
-void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s) {
-    hipLaunchKernelGGL(fock_finalize, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, X, cons, G, n);
+void launch_fock_finalize(const unsigned long long* X, int copies, const double* cons, double* G, int n,
+                          unsigned long long* flag, hipStream_t s) {
+    hipLaunchKernelGGL(fock_finalize, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, X, copies, cons, G, n,
+                       flag);
 }
 
 }  // namespace qk

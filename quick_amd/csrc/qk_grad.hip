@@ -353,9 +353,15 @@ __global__ void __launch_bounds__(kGradTeam, kMinWaves) grad_kernel(const GradAr
 }
 
 // grad <- grad + G * 2^-44
-__global__ void grad_finalize(const unsigned long long* __restrict__ G, double* __restrict__ grad, int m) {
+// grad += G (2^-44 units); a sum within a factor of two of the int64 range
+// (|dE/dR| of 2^18 Eh/bohr and more) raises the range flag
+__global__ void grad_finalize(const unsigned long long* __restrict__ G, double* __restrict__ grad, int m,
+                              unsigned long long* __restrict__ flag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m) grad[k] += (double)(long long)G[k] * 0x1p-44;
+    if (k >= m) return;
+    const long long s = (long long)G[k];
+    if (s >= (1LL << 62) || s <= -(1LL << 62)) atomicOr(flag, 1ull);
+    grad[k] += (double)s * 0x1p-44;
 }
 
 // four wavefronts per quartet for the classes with the largest node programs
@@ -378,7 +384,10 @@ static int launch_team(const GradArgs& a, const ClassWork& w, const int* dtab, s
         hipFuncSetAttribute(reinterpret_cast<const void*>(&grad_kernel<TEAM, R, W>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL((grad_kernel<TEAM, R, W>), dim3(grid), dim3(TEAM), lds, s, a, w, dtab);
+    static const long long cap =
+        scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&grad_kernel<TEAM, R, W>)), TEAM / 64);
+    hipLaunchKernelGGL((grad_kernel<TEAM, R, W>), dim3((int)std::min<long long>(grid, cap)), dim3(TEAM), lds, s, a, w,
+                       dtab);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -420,8 +429,8 @@ int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bo
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s) {
-    hipLaunchKernelGGL(grad_finalize, dim3((m + 255) / 256), dim3(256), 0, s, G, grad, m);
+void launch_grad_finalize(const unsigned long long* G, double* grad, int m, unsigned long long* flag, hipStream_t s) {
+    hipLaunchKernelGGL(grad_finalize, dim3((m + 255) / 256), dim3(256), 0, s, G, grad, m, flag);
 }
 
 }  // namespace qk

@@ -35,14 +35,15 @@ void launch_scale_density(const double* D, const double* cons, double* Dp, float
 size_t grad_lds_bytes(const int* hdr, bool fp32);
 int launch_grad_class(const GradArgs& a, const ClassWork& w, const int* hdr, const int* dtab, size_t lds, int grid,
                       bool fp32, bool four_waves, hipStream_t s);
-void launch_grad_finalize(const unsigned long long* G, double* grad, int m, hipStream_t s);
+void launch_grad_finalize(const unsigned long long* G, double* grad, int m, unsigned long long* flag, hipStream_t s);
 bool grad_class_is_light(int k);
 int launch_grad_light(int k, const GradArgs& a, const ClassWork& w, int grid, bool fp32, hipStream_t s);
 void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, double* cm, int n, int ns,
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
 void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
-void launch_fock_finalize(const unsigned long long* X, const double* cons, double* G, int n, hipStream_t s);
+void launch_fock_finalize(const unsigned long long* X, int copies, const double* cons, double* G, int n,
+                          unsigned long long* flag, hipStream_t s);
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
                     const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
                     double* ymax, double* pmax, hipStream_t s);
@@ -322,13 +323,18 @@ struct qk_ctx {
     int cls_ms_n[kSlots] = {};
 
     // reference-ABI state
-    int method = 0;
     double abi_tau = 1e-11, abi_prim_limit = 1e-20;
-    std::vector<double> abi_cutmatrix;
+    DBuf<double> d_abi_cm;   // cutMatrix of the last gpu_upload_cutoff_
     bool abi_have_cutmatrix = false;
     bool abi_have_dense = false;
     double abi_grad_cutoff = 1e-11;
+    std::vector<double> abi_grad_seed;
+    bool abi_have_grad_seed = false;
 };
+
+extern "C" long long qk_scratch_grid_cap(long long lane_bytes, int waves_per_block) {
+    return scratch_grid_cap(lane_bytes, waves_per_block);
+}
 
 extern "C" void qk_default_opts(qk_opts* o) {
     o->tau = 1e-11;
@@ -521,7 +527,7 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         c->d_sh_f0.upload(sh_f0) || c->d_sh_nf.upload(sh_nf))
         return -1;
     size_t nn = (size_t)nbasis * nbasis;
-    if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
+    if (c->d_Dp.alloc(nn) || c->d_X.alloc((size_t)xtri_size(nbasis) * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
     if (build_all_pairs(c)) return -1;
     c->have_system = true;
     c->have_schwarz = false;
@@ -783,6 +789,7 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
     a.cutmat = c->d_cm.p;
     a.X = c->d_X.p;
     a.counters = c->d_counters.p;
+    a.xcopies = kXCopies;
     a.n = c->nbasis;
     a.nshell = c->nshell;
     a.tau = o->tau;
@@ -833,7 +840,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
-    launch_fill_u64(c->d_X.p, (long long)n * n * kXCopies, s);
+    launch_fill_u64(c->d_X.p, xtri_size(n) * kXCopies, s);
     launch_fill_u64(c->d_counters.p, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o);
@@ -847,9 +854,8 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     // All classes run one after another on the caller's stream (one hardware
     // queue: kernels that spill to scratch never run concurrently with others).
     // Kernels that spill get a grid small enough that their private-segment
-    // backing (per-lane scratch x 64 x resident waves) stays under
-    // kScratchBudget; the grid-stride loop over tiles covers the rest.
-    constexpr double kScratchBudget = 256.0 * 1024 * 1024;
+    // backing stays under kScratchBudget (scratch_grid_cap); the grid-stride
+    // loop over tiles covers the rest.
     constexpr size_t kRowsLdsMax = 64 * 1024;   // LDS for one bra's exchange rows
     for (int k : order) {
         ClassHost& CH = c->classes[k];
@@ -872,9 +878,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
                 launch_team_class(k, a, w, grid, s);
             } else {
-                long long cap = 256LL * 16;
-                const size_t scratch = fock_class_scratch(k);
-                if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+                const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch(k), 4));
                 if (use_runs && CH.compact) {
                     // density screen first: every lane of the class kernel then has a quartet
                     launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
@@ -896,9 +900,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         }
         if (mixed && !CH.tiles32.empty()) {
             a.counters = c->d_counters.p + 2 * kNumQuartetClasses + 2 * k;
-            long long cap = 256LL * 16;
-            const size_t scratch = fock_class_scratch_f32(k);
-            if (scratch > 0) cap = std::max(1LL, (long long)(kScratchBudget / (double(scratch) * 256.0)));
+            const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch_f32(k), 4));
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
             ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
@@ -928,7 +930,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[slot] = c->timing;
         }
     }
-    launch_fock_finalize(c->d_X.p, c->d_cons.p, dG, n, s);
+    launch_fock_finalize(c->d_X.p, kXCopies, c->d_cons.p, dG, n, c->d_counters.p + kRangeFlag, s);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1063,6 +1065,27 @@ extern "C" int qk_last_stats(qk_ctx* c, qk_stats* s) {
     return 0;
 }
 
+// Reads a range flag the finalize kernels raise (waits for the work before it).
+static int check_range(const unsigned long long* dflag, const char* what) {
+    unsigned long long f = 0;
+    if (!dflag) return 0;
+    HIPCHK(hipMemcpy(&f, dflag, sizeof(f), hipMemcpyDeviceToHost));
+    if (f) {
+        set_err("%s: the int64 fixed-point accumulator came within a factor of two of its range "
+                "(|G| >= 8192 or |dE/dR| >= 2^18); the result is not trustworthy", what);
+        return -5;
+    }
+    return 0;
+}
+
+extern "C" int qk_check_range(qk_ctx* c) {
+    if (!c) { set_err("null context"); return -1; }
+    (void)hipSetDevice(c->device);
+    if (check_range(c->d_counters.p ? c->d_counters.p + kRangeFlag : nullptr, "last Fock build")) return -5;
+    if (check_range(c->d_gcounters.p ? c->d_gcounters.p + kGradRangeFlag : nullptr, "last gradient")) return -5;
+    return 0;
+}
+
 extern "C" int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o) {
     if (!c) { set_err("null context"); return -1; }
     qk_opts d;
@@ -1074,6 +1097,7 @@ extern "C" int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts
     if (qk_fock_device(c, c->d_D.p, c->d_G.p, o, nullptr)) return -1;
     std::vector<double> G(nn);
     HIPCHK(hipMemcpy(G.data(), c->d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (check_range(c->d_counters.p + kRangeFlag, "qk_fock_host")) return -5;
     for (size_t k = 0; k < nn; ++k) F[k] += G[k];
     return 0;
 }
@@ -1257,7 +1281,7 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
                 return -1;
         }
     }
-    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(4 * kNumQuartetClasses)) return -1;
+    if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(kNumGradCounters)) return -1;
     c->grad_prep_cutoff = cutoff;
     c->grad_prep_precision = o->precision;
     c->grad_prep_thr = o->fp32_threshold;
@@ -1298,7 +1322,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
     launch_fill_u64(c->d_gacc.p, 3LL * c->natom, s);
-    launch_fill_u64(c->d_gcounters.p, 4LL * kNumQuartetClasses, s);
+    launch_fill_u64(c->d_gcounters.p, kNumGradCounters, s);
     HIPCHK(hipGetLastError());
     GradArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
@@ -1378,7 +1402,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[slot] = c->timing;
         }
     }
-    launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, s);
+    launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, c->d_gcounters.p + kGradRangeFlag, s);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1402,7 +1426,7 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
     HIPCHK(hipMemcpy(c->d_grad.p, grad, m * sizeof(double), hipMemcpyHostToDevice));
     if (grad_enqueue(c, c->d_D.p, nullptr, c->d_grad.p, o, nullptr)) return -1;
     HIPCHK(hipMemcpy(grad, c->d_grad.p, m * sizeof(double), hipMemcpyDeviceToHost));
-    return 0;
+    return check_range(c->d_gcounters.p + kGradRangeFlag, "qk_grad_host");
 }
 
 // the forward/backward pair of the test harness (same operations as the
@@ -1456,7 +1480,11 @@ static int g_dev_req = -1;
 static bool g_started = false;
 static int g_natom = 0, g_nbasis = 0;
 static std::vector<double> g_xyz;
-static double* g_dense_host = nullptr;
+static std::vector<double> g_atom_chg;   // nuclear charges (gpu_upload_atom_and_chg_)
+static std::vector<int> g_atom_type;     // atomic numbers
+// gpu_upload_method_ may come before gpu_init_ (QUICK calls it per operator
+// build); the method is kept here and read by every build
+static int g_method = 0;
 
 extern "C" qk_ctx* qk_global_context(void) { return g_ctx; }
 
@@ -1465,6 +1493,8 @@ static void fatal(const char* what) {
     fflush(stderr);
     std::abort();
 }
+
+static double hybrid_of_method(int m) { return m == 0 ? 1.0 : (m == 1 ? 0.2 : 0.0); }
 
 extern "C" void gpu_startup_(void) { g_started = true; }
 extern "C" void gpu_set_device_(int* id) { g_dev_req = id ? *id : -1; }
@@ -1500,7 +1530,10 @@ extern "C" void gpu_setup_(int* natom, int* nbasis, int*, int*, int*, int*) {
     g_nbasis = *nbasis;
 }
 extern "C" void gpu_upload_xyz_(double* xyz) { g_xyz.assign(xyz, xyz + 3 * (size_t)g_natom); }
-extern "C" void gpu_upload_atom_and_chg_(int*, double*) {}
+extern "C" void gpu_upload_atom_and_chg_(int* atom_type, double* chg) {
+    g_atom_chg.assign(chg, chg + (size_t)g_natom);
+    g_atom_type.assign(atom_type, atom_type + (size_t)g_natom);
+}
 
 extern "C" void gpu_upload_basis_(int* nshell, int*, int*, int* jbasis, int*, int*, int*, double*, double*, int*,
                                   int*, int*, int*, int*, int* kstart, int* katom, int* ktype, int* kprim, int*,
@@ -1515,7 +1548,7 @@ extern "C" void gpu_upload_cutoff_matrix_(double* Y, double* cutPrim) {
     if (!g_ctx || qk_set_schwarz(g_ctx, Y, cutPrim)) fatal("gpu_upload_cutoff_matrix");
 }
 extern "C" void gpu_upload_method_(int* m) {
-    if (g_ctx) g_ctx->method = *m;
+    g_method = *m;
 }
 extern "C" void gpu_upload_calculated_(double*, double*, double*, double* dense) {
     if (!g_ctx) fatal("gpu_upload_calculated (no context)");
@@ -1525,15 +1558,28 @@ extern "C" void gpu_upload_calculated_(double*, double*, double*, double* dense)
         set_err("density upload failed");
         fatal("gpu_upload_calculated");
     }
-    g_dense_host = dense;
     g_ctx->abi_have_dense = true;
 }
+// The caller's cutMatrix goes into a device buffer kept with the context (one
+// allocation per system size); the next gpu_get2e_ / gpu_grad_ / gpu_getxc_
+// consumes it, as the reference frees it after the build (gpu.cu:1857-1861).
 extern "C" void gpu_upload_cutoff_(double* cutMatrix, double* integralCutoff, double* primLimit, double*) {
     if (!g_ctx) fatal("gpu_upload_cutoff (no context)");
-    g_ctx->abi_tau = *integralCutoff;
-    g_ctx->abi_prim_limit = *primLimit;
-    g_ctx->abi_cutmatrix.assign(cutMatrix, cutMatrix + (size_t)g_ctx->nshell * g_ctx->nshell);
-    g_ctx->abi_have_cutmatrix = true;
+    qk_ctx* c = g_ctx;
+    c->abi_tau = *integralCutoff;
+    c->abi_prim_limit = *primLimit;
+    const size_t m = (size_t)c->nshell * c->nshell;
+    if (c->d_abi_cm.alloc(m) ||
+        hipMemcpy(c->d_abi_cm.p, cutMatrix, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        set_err("cutMatrix upload failed");
+        fatal("gpu_upload_cutoff");
+    }
+    c->abi_have_cutmatrix = true;
+}
+// the per-cycle uploads are consumed by the call that reads them
+static void abi_consume(qk_ctx* c) {
+    c->abi_have_dense = false;
+    c->abi_have_cutmatrix = false;
 }
 extern "C" void gpu_get2e_(double* o) {
     qk_ctx* c = g_ctx;
@@ -1545,14 +1591,9 @@ extern "C" void gpu_get2e_(double* o) {
     qk_default_opts(&op);
     op.tau = c->abi_tau;
     op.prim_limit = c->abi_prim_limit;
-    op.hybrid = c->method == 0 ? 1.0 : (c->method == 1 ? 0.2 : 0.0);
+    op.hybrid = hybrid_of_method(g_method);
     const size_t nn = (size_t)c->nbasis * c->nbasis;
-    DBuf<double> dcm;
-    const double* cmp = nullptr;
-    if (c->abi_have_cutmatrix) {
-        if (dcm.upload(c->abi_cutmatrix)) fatal("gpu_get2e");
-        cmp = dcm.p;
-    }
+    const double* cmp = c->abi_have_cutmatrix ? c->d_abi_cm.p : nullptr;
     (void)hipSetDevice(c->device);
     if (fock_enqueue(c, c->d_D.p, cmp, c->d_G.p, &op, nullptr)) fatal("gpu_get2e");
     std::vector<double> G(nn);
@@ -1560,38 +1601,49 @@ extern "C" void gpu_get2e_(double* o) {
         set_err("operator download failed");
         fatal("gpu_get2e");
     }
+    if (check_range(c->d_counters.p + kRangeFlag, "gpu_get2e")) fatal("gpu_get2e");
     for (size_t k = 0; k < nn; ++k) o[k] += G[k];
-    // per-cycle state is consumed by the build (gpu.cu:1857-1861)
-    c->abi_have_dense = false;
-    c->abi_have_cutmatrix = false;
+    abi_consume(c);
 }
-extern "C" void gpu_upload_grad_(double*, double* gradCutoff) {
+// Keeps the gradient seed (the caller's partial gradient, 3 natom) and the
+// gradient cutoff; the next gpu_grad_ writes seed + four-centre term
+// (gpu.cu:1393-1443, 1481-1530).
+extern "C" void gpu_upload_grad_(double* grad, double* gradCutoff) {
     if (!g_ctx) fatal("gpu_upload_grad (no context)");
+    g_ctx->abi_grad_seed.assign(grad, grad + 3 * (size_t)g_ctx->natom);
     g_ctx->abi_grad_cutoff = *gradCutoff;
+    g_ctx->abi_have_grad_seed = true;
 }
-// grad <- grad + the four-centre gradient of the uploaded density (gpu.cu:1481-1530)
+// grad <- seed + the four-centre gradient of the uploaded density.  Screening
+// follows the reference CPU path (gradient.f90:289-316, shell.f90:1941): the
+// quartet test and the primitive test both use the gradient cutoff from
+// gpu_upload_grad_.  (The reference GPU kernel screens with integralCutoff and
+// primLimit instead; QUICK's CUTOFF= keyword sets integralCutoff = gradCutoff.)
 extern "C" void gpu_grad_(double* grad) {
     qk_ctx* c = g_ctx;
     if (!c || !c->abi_have_dense) {
         set_err("gpu_grad: density not uploaded (call gpu_upload_calculated first)");
         fatal("gpu_grad");
     }
+    if (!c->abi_have_grad_seed) {
+        set_err("gpu_grad: gradient not uploaded (call gpu_upload_grad first)");
+        fatal("gpu_grad");
+    }
     qk_opts op;
     qk_default_opts(&op);
     op.tau = c->abi_grad_cutoff;
-    op.hybrid = c->method == 0 ? 1.0 : (c->method == 1 ? 0.2 : 0.0);
-    DBuf<double> dcm;
-    const double* cmp = nullptr;
-    if (c->abi_have_cutmatrix) {
-        if (dcm.upload(c->abi_cutmatrix)) fatal("gpu_grad");
-        cmp = dcm.p;
-    }
+    op.hybrid = hybrid_of_method(g_method);
+    const double* cmp = c->abi_have_cutmatrix ? c->d_abi_cm.p : nullptr;
     (void)hipSetDevice(c->device);
     const size_t m = 3 * (size_t)c->natom;
-    if (c->d_grad.alloc(m) || hipMemcpy(c->d_grad.p, grad, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    if (c->d_grad.alloc(m) ||
+        hipMemcpy(c->d_grad.p, c->abi_grad_seed.data(), m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         fatal("gpu_grad");
     if (grad_enqueue(c, c->d_D.p, cmp, c->d_grad.p, &op, nullptr)) fatal("gpu_grad");
     if (hipMemcpy(grad, c->d_grad.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) fatal("gpu_grad");
+    if (check_range(c->d_gcounters.p + kGradRangeFlag, "gpu_grad")) fatal("gpu_grad");
+    abi_consume(c);
+    c->abi_have_grad_seed = false;
 }
 extern "C" void gpu_getxc_(int*, double*, double*, double*, double*, double*) {
     set_err("gpu_getxc: the XC step is not provided by this library");

@@ -11,13 +11,48 @@ namespace qk {
 constexpr int kNumPairClasses = 6;     // ss ps pp ds dp dd (la >= lb)
 constexpr int kNumQuartetClasses = 21;  // bra pair class >= ket pair class
 // device counters: per class (surviving quartets, primitive quartets) for the
-// FP64 kernels at [2k, 2k+1] and for the FP32 kernels at [2K+2k, 2K+2k+1]
-constexpr int kNumCounters = 4 * kNumQuartetClasses + 2;
-// The operator accumulator X is kept in kXCopies copies (n x n int64 each);
-// block b adds into copy b % kXCopies, and the copies are summed at the end.
-// Consecutive blocks run on different XCDs, so each copy's atomics come
-// mostly from one XCD and concurrent atomics spread over more addresses.
+// FP64 kernels at [2k, 2k+1] and for the FP32 kernels at [2K+2k, 2K+2k+1];
+// the last one is the range flag of the fixed-point accumulator (set by the
+// finalize kernel when a sum comes within a factor of two of wrapping)
+constexpr int kNumCounters = 4 * kNumQuartetClasses + 3;
+constexpr int kRangeFlag = kNumCounters - 1;
+// gradient counters: per class (quartets, primitive quartets), FP64 then FP32,
+// then the gradient accumulator's range flag
+constexpr int kNumGradCounters = 4 * kNumQuartetClasses + 1;
+constexpr int kGradRangeFlag = kNumGradCounters - 1;
+// The operator accumulator X holds the unordered pairs {i, j} of basis
+// functions as a packed lower triangle (n(n+1)/2 int64, see xtri): G depends on
+// X + X^T only, so a contribution to X[i][j] or X[j][i] lands in one slot.  It
+// is kept in up to kXCopies copies; block b adds into copy b % copies, and the
+// copies are summed at the end.  Consecutive blocks run on different XCDs, so
+// each copy's atomics come mostly from one XCD (at 800 functions a copy is
+// 2.6 MB, within one XCD's 4 MB L2).
 constexpr int kXCopies = 8;
+__host__ __device__ inline long long xtri_size(int n) { return (long long)n * (n + 1) / 2; }
+// slot of the unordered pair {i, j} in the packed lower triangle
+__host__ __device__ __forceinline__ long long xtri(int i, int j) {
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    return (long long)hi * (hi + 1) / 2 + lo;
+}
+// Scratch (private segment) backing of a dispatch: the runtime sizes it as the
+// kernel's per-lane scratch x 64 lanes x the waves the grid can have in flight
+// (at most 8192 on 256 CUs).  Kernels that spill get their grid bounded so that
+// this stays under kScratchBudget; their grid-stride loops cover the rest.
+constexpr double kScratchBudget = 256.0 * 1024 * 1024;
+constexpr long long kMaxWavesInFlight = 256LL * 32;
+// largest grid (blocks of waves_per_block wavefronts) whose scratch backing stays
+// under kScratchBudget for a kernel with lane_bytes of scratch per lane
+inline long long scratch_grid_cap(long long lane_bytes, int waves_per_block) {
+    if (lane_bytes <= 0) return 1LL << 40;
+    const long long c = (long long)(kScratchBudget / (double(lane_bytes) * 64.0 * waves_per_block));
+    return c < 1 ? 1 : c;
+}
+// per-lane scratch of a kernel (0 when the query fails)
+inline long long kernel_scratch(const void* k) {
+    hipFuncAttributes at{};
+    return hipFuncGetAttributes(&at, k) == hipSuccess ? (long long)at.localSizeBytes : 0;
+}
+
 constexpr int kFixedShift = 48;         // Fock accumulator: int64 in units of 2^-48
 constexpr int kGradShift = 44;          // gradient accumulator: int64 in units of 2^-44
 
@@ -172,8 +207,9 @@ struct FockArgs {
     const double* xyz;        // 3 x natom
     const double* Dp;         // cons-scaled density, n x n
     const double* cutmat;     // nshell x nshell, max |D| over shell blocks
-    unsigned long long* X;    // n x n fixed-point accumulator
+    unsigned long long* X;    // fixed-point accumulator: xcopies packed triangles (xtri)
     unsigned long long* counters;  // [0] surviving quartets, [1] primitive quartets
+    int xcopies;              // copies of the accumulator (1 .. kXCopies)
     int n;
     int nshell;
     double tau;

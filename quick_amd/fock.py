@@ -70,6 +70,8 @@ def lib() -> C.CDLL:
         L.qk_grad_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_grad_host.argtypes = [P, dp, dp, C.POINTER(QkOpts)]
         L.qk_last_stats.argtypes = [P, C.POINTER(QkStats)]
+        L.qk_check_range.argtypes = [P]
+        L.qk_check_range.restype = C.c_int
         L.qk_num_classes.restype = C.c_int
         L.qk_class_info.argtypes = [P, C.c_int, ip, C.POINTER(C.c_longlong)]
         L.qk_set_timing.argtypes = [P, C.c_int]

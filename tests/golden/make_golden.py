@@ -12,7 +12,13 @@ system, one compressed ``.npz`` beside this script:
     returned by hfoperator with a zero one-electron part (hfoperator.f90:24-138);
   * optionally the four-centre gradient term ``grad4c`` (gradient.f90:261-320).
 
-usage (repository root):  python tests/golden/make_golden.py [system ...]
+With ``--scf`` it instead runs the driver's ``scfgrad`` mode and stores, as
+``<system>_scf.npz``, the reference at a converged RHF density: the occupied
+orbitals (D = 2 C_occ C_occ^T), their energies, the total energy and its parts,
+H_core, S and G(D) (sampled for systems above 400 functions), the full analytic
+gradient of hfgrad and its four-centre term.
+
+usage (repository root):  python tests/golden/make_golden.py [--scf] [system ...]
 """
 from __future__ import annotations
 
@@ -40,7 +46,77 @@ SYSTEMS = {
     "ethane": (True, True),
     "water4": (True, True),
     "water32": (True, False),   # full benchmark size: ~10 min of reference CPU time
+    "water64": (True, False),   # 1600 functions: ~1 h of reference CPU time
 }
+SCF_SYSTEMS = ["h2o", "water4", "icosane", "water32"]
+
+
+def _read_setup(od: Path, syms) -> dict:
+    natom, nbasis, nshell, _, jbasis, maxprim = np.fromfile(od / "sizes.bin", np.int32)
+    rd = lambda n, dt=np.float64: np.fromfile(od / f"{n}.bin", dt)
+    return dict(
+        natom=natom, nbasis=nbasis, nshell=nshell, jbasis=jbasis, maxprim=maxprim,
+        symbols=np.array(syms), xyz=rd("xyz").reshape(3, natom, order="F"),
+        kstart=rd("kstart", np.int32), kprim=rd("kprim", np.int32),
+        katom=rd("katom", np.int32), ktype=rd("ktype", np.int32),
+        Qstart=rd("Qstart", np.int32), Qfinal=rd("Qfinal", np.int32),
+        Qsbasis=rd("Qsbasis", np.int32).reshape(nshell, 4, order="F"),
+        Qfbasis=rd("Qfbasis", np.int32).reshape(nshell, 4, order="F"),
+        ksumtype=rd("ksumtype", np.int32),
+        KLMN=rd("KLMN", np.int32).reshape(3, nbasis, order="F"),
+        cons=rd("cons"),
+        gccoeff=rd("gccoeff").reshape(maxprim, nbasis, order="F"),
+        gcexpo=rd("gcexpo").reshape(maxprim, nbasis, order="F"),
+        Ycutoff=rd("Ycutoff").reshape(nshell, nshell, order="F"),
+        cutprim=rd("cutprim").reshape(jbasis, jbasis, order="F"),
+    )
+
+
+def _sample(out: dict, name: str, M: np.ndarray, rng_seed: int = 12345) -> None:
+    """Seeded element samples, the diagonal and row norms of an n x n matrix."""
+    n = M.shape[0]
+    rng = np.random.default_rng(rng_seed)
+    ii, jj = rng.integers(0, n, 40000), rng.integers(0, n, 40000)
+    out.update({f"{name}_sample_i": ii.astype(np.int32), f"{name}_sample_j": jj.astype(np.int32),
+                f"{name}_sample": M[ii, jj], f"{name}_diag": np.diag(M).copy(),
+                f"{name}_rownorm": np.linalg.norm(M, axis=1)})
+
+
+def run_scf(name: str) -> Path:
+    """The reference at a converged RHF density (driver mode scfgrad)."""
+    syms, xyz = small_system(name)
+    env = dict(os.environ, QUICK_BASIS=BASIS_DIR)
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "job.in").write_text(to_quick_input(syms, xyz, KEYWORDS + " OPT"))
+        od = td / "scf"
+        od.mkdir()
+        subprocess.run(f"ulimit -s unlimited && {DRIVER} job.in scfgrad none {od}",
+                       shell=True, check=True, cwd=td, env=env, stdout=subprocess.DEVNULL)
+        out = _read_setup(od, syms)
+        n, natom = int(out["nbasis"]), int(out["natom"])
+        rd = lambda f: np.fromfile(od / f"{f}.bin")
+        E = rd("energy")
+        nocc = int(sum({"H": 1, "C": 6, "N": 7, "O": 8}[x] for x in syms)) // 2
+        C = rd("C").reshape(n, n, order="F")[:, :nocc].copy()
+        D, H, S, G = (rd(f).reshape(n, n, order="F") for f in ("D", "H", "S", "G"))
+        out.update(C_occ=C, eps=rd("eps")[:nocc + 1], E_total=np.array(E[0]), E_el=np.array(E[1]),
+                   E_nn=np.array(E[2]), scf_cycles=np.array(E[3]), scf_rms=np.array(E[4]),
+                   time_scf=np.array(E[5]), E2=np.array(0.5 * np.sum(D * G)), E1=np.array(np.sum(D * H)),
+                   D_ref_minus_CCt=np.array(np.abs(D - 2.0 * C @ C.T).max()),
+                   grad_total=rd("gradtotal").reshape(3, natom, order="F"),
+                   grad4c=rd("grad4c").reshape(3, natom, order="F"),
+                   time_grad=np.array(rd("time_grad")[0]))
+    if n > 400:
+        for k in ("cutprim", "gccoeff", "gcexpo"):
+            out.pop(k)
+        _sample(out, "G", G)
+        _sample(out, "H", H, 54321)
+    else:
+        out.update(G=G, H=H, S=S)
+    dst = Path(__file__).with_name(f"{name}_scf.npz")
+    np.savez_compressed(dst, **out)
+    return dst
 
 
 def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
@@ -60,26 +136,9 @@ def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
                            shell=True, check=True, cwd=td, env=env,
                            stdout=subprocess.DEVNULL)
             if mode == "setup":
-                natom, nbasis, nshell, _, jbasis, maxprim = np.fromfile(od / "sizes.bin", np.int32)
-                rd = lambda n, dt=np.float64: np.fromfile(od / f"{n}.bin", dt)
-                out.update(
-                    natom=natom, nbasis=nbasis, nshell=nshell, jbasis=jbasis, maxprim=maxprim,
-                    symbols=np.array(syms), xyz=rd("xyz").reshape(3, natom, order="F"),
-                    kstart=rd("kstart", np.int32), kprim=rd("kprim", np.int32),
-                    katom=rd("katom", np.int32), ktype=rd("ktype", np.int32),
-                    Qstart=rd("Qstart", np.int32), Qfinal=rd("Qfinal", np.int32),
-                    Qsbasis=rd("Qsbasis", np.int32).reshape(nshell, 4, order="F"),
-                    Qfbasis=rd("Qfbasis", np.int32).reshape(nshell, 4, order="F"),
-                    ksumtype=rd("ksumtype", np.int32),
-                    KLMN=rd("KLMN", np.int32).reshape(3, nbasis, order="F"),
-                    cons=rd("cons"),
-                    gccoeff=rd("gccoeff").reshape(maxprim, nbasis, order="F"),
-                    gcexpo=rd("gcexpo").reshape(maxprim, nbasis, order="F"),
-                    Ycutoff=rd("Ycutoff").reshape(nshell, nshell, order="F"),
-                    cutprim=rd("cutprim").reshape(jbasis, jbasis, order="F"),
-                )
+                out.update(_read_setup(od, syms))
                 out["D"] = synthetic_density(out["xyz"], out["katom"], out["ksumtype"],
-                                             int(nbasis), seed)
+                                             int(out["nbasis"]), seed)
             elif mode == "fock":
                 out["G"] = np.fromfile(od / "G.bin").reshape(out["nbasis"], out["nbasis"], order="F")
                 out["time_fock"] = np.fromfile(od / "time_fock.bin")[0]
@@ -93,19 +152,24 @@ def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
         G, D = out.pop("G"), out.pop("D")
         for k in ("cutprim", "gccoeff", "gcexpo"):
             out.pop(k)
-        n = G.shape[0]
-        rng = np.random.default_rng(12345)
-        ii, jj = rng.integers(0, n, 40000), rng.integers(0, n, 40000)
-        out.update(G_sample_i=ii.astype(np.int32), G_sample_j=jj.astype(np.int32), G_sample=G[ii, jj],
-                   G_diag=np.diag(G).copy(), G_rownorm=np.linalg.norm(G, axis=1),
-                   E2=np.array(0.5 * np.sum(D * G)), density_seed=np.array(seed))
+        _sample(out, "G", G)
+        out.update(E2=np.array(0.5 * np.sum(D * G)), density_seed=np.array(seed))
     dst = Path(__file__).with_name(f"{name}.npz")
     np.savez_compressed(dst, **out)
     return dst
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or list(SYSTEMS)
+    args = sys.argv[1:]
+    if args and args[0] == "--scf":
+        for n in args[1:] or SCF_SYSTEMS:
+            p = run_scf(n)
+            z = np.load(p)
+            print(p.name, "nbasis", int(z["nbasis"]), "E", float(z["E_total"]), "cycles", int(z["scf_cycles"]),
+                  "scf s", float(z["time_scf"]), "grad s", float(z["time_grad"]),
+                  f"{p.stat().st_size/1024:.0f} KiB")
+        sys.exit(0)
+    names = args or list(SYSTEMS)
     for n in names:
         f, g = SYSTEMS[n]
         p = run(n, f, g)

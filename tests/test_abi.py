@@ -46,3 +46,19 @@ def test_no_device_is_an_error_not_a_crash():
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("NULL")
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="library not built")
+def test_abi_build_before_upload_is_fatal_with_message():
+    """gpu_get2e_ before any upload: the library aborts with a message, as the
+    reference exits (gpu_common.h:142-146); a child process with no visible
+    device, so that nothing reaches a GPU."""
+    import os
+    import subprocess
+    import sys
+    code = ("import ctypes, numpy as np; L = ctypes.CDLL(%r); o = np.zeros(4); "
+            "L.gpu_get2e_(o.ctypes.data_as(ctypes.c_void_p))" % str(LIB))
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "gpu_get2e: density not uploaded" in r.stderr

@@ -76,3 +76,23 @@ def test_contraction_longer_than_staged_is_rejected():
     b.kprim[0] = 9
     with pytest.raises(RuntimeError, match="at most 8"):
         FockEngine(b)
+
+
+def test_fixed_point_range_guard(golden):
+    """A density scaled so that |G| reaches 2e4 (the accumulator's range is about
+    3.3e4 for off-diagonal elements): the build reports an error instead of
+    returning a silently wrapped operator; at the normal scale the check passes."""
+    from quick_amd.fock import FockEngine, lib
+    print("[gpu-test] fixed-point range guard water2", flush=True)
+    fx = golden("water2")
+    eng = FockEngine(fx)
+    eng.set_schwarz(fx.Ycutoff, fx.cutprim)
+    G = eng.fock(fx.D)
+    assert lib().qk_check_range(eng.h) == 0
+    scale = 2.0e4 / np.abs(G).max()
+    with pytest.raises(RuntimeError, match="fixed-point"):
+        eng.fock(fx.D * scale)
+    assert lib().qk_check_range(eng.h) == -5
+    G2 = eng.fock(fx.D)   # the next build clears the flag
+    assert np.array_equal(G2, G)
+    assert lib().qk_check_range(eng.h) == 0

@@ -136,50 +136,6 @@ def test_water32_translation_and_permutation_invariance():
     assert np.abs(G3 - G1[np.ix_(fperm, fperm)]).max() <= 1e-9
 
 
-def test_abi_reference_call_sequence(golden):
-    """The Fortran call sequence of main.f90/hfoperator.f90 through the gpu_*_ symbols."""
-    from quick_amd.fock import lib
-    _banner("reference ABI h2o")
-    fx = golden("h2o")
-    L = lib()
-    i = lambda v: ctypes.byref(ctypes.c_int(v))
-    d = lambda v: ctypes.byref(ctypes.c_double(v))
-    arr = lambda a, t: np.ascontiguousarray(np.asarray(a, t).ravel(order="F"))
-    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    L.gpu_startup_()
-    L.gpu_set_device_(i(-1))
-    L.gpu_init_()
-    L.gpu_setup_(i(int(fx.natom)), i(int(fx.nbasis)), i(10), i(1), i(0), i(0))
-    xyz = arr(fx.xyz, np.float64)
-    L.gpu_upload_xyz_(P(xyz))
-    ints = {k: arr(getattr(fx, k), np.int32) for k in ("kstart", "katom", "ktype", "kprim", "ksumtype",
-                                                       "Qstart", "Qfinal", "Qsbasis", "Qfbasis", "KLMN")}
-    dbl = {k: arr(getattr(fx, k), np.float64) for k in ("gccoeff", "cons", "gcexpo")}
-    z = np.zeros(16, np.int32)
-    zd = np.zeros(16)
-    L.gpu_upload_basis_(i(int(fx.nshell)), i(int(fx.jbasis)), i(int(fx.nshell)), i(int(fx.jbasis)), i(10),
-                        P(z), P(z), P(zd), P(zd), P(z), P(z), P(z), P(z), P(z),
-                        P(ints["kstart"]), P(ints["katom"]), P(ints["ktype"]), P(ints["kprim"]), P(z),
-                        P(ints["ksumtype"]), P(z), P(ints["Qstart"]), P(ints["Qfinal"]), P(ints["Qsbasis"]),
-                        P(ints["Qfbasis"]), P(dbl["gccoeff"]), P(dbl["cons"]), P(dbl["gcexpo"]), P(ints["KLMN"]))
-    Yc, cp = arr(fx.Ycutoff, np.float64), arr(fx.cutprim, np.float64)
-    L.gpu_upload_cutoff_matrix_(P(Yc), P(cp))
-    o = Oracle(fx)
-    cm = arr(o.density_cutoff(fx.D), np.float64)
-    H = np.random.default_rng(5).normal(size=(fx.nbasis, fx.nbasis))
-    H = H + H.T
-    O = arr(H, np.float64)
-    dense = arr(fx.D, np.float64)
-    L.gpu_upload_method_(i(0))
-    L.gpu_upload_calculated_(P(O), P(zd), P(zd), P(dense))
-    L.gpu_upload_cutoff_(P(cm), d(1e-11), d(1e-20))
-    L.gpu_get2e_(P(O))
-    F = O.reshape(fx.nbasis, fx.nbasis, order="F")
-    assert np.abs(F - (H + fx.G)).max() <= TOL_F
-    L.gpu_cleanup_()
-    L.gpu_shutdown_()
-
-
 def test_water32_full_size_matches_reference_samples(golden):
     """(H2O)32, 800 bf: the benchmark system against the reference CPU path's
     own full build (40k seeded element samples, diagonal, row norms, E2)."""
