@@ -40,7 +40,7 @@ void gpu_get_device_info_(int* gpu_dev_count, int* gpu_dev_id, int* gpu_dev_mem,
 void gpu_setup_(int* natom, int* nbasis, int* nElec, int* imult, int* molchg, int* iAtomType);
 /* gpu.cu:283  coordinates, 3 x natom, bohr */
 void gpu_upload_xyz_(double* atom_xyz);
-/* gpu.cu:334  atom types and charges (kept for the XC step; unused by the ERIs) */
+/* gpu.cu:334  atom types (atomic numbers) and charges, kept for the XC step */
 void gpu_upload_atom_and_chg_(int* atom_type, double* atom_chg);
 /* gpu.cu:1070 basis: QUICK's shell tables (quick_basis_module) */
 void gpu_upload_basis_(int* nshell, int* nprim, int* jshell, int* jbasis, int* maxcontract,
@@ -72,7 +72,12 @@ void gpu_upload_grad_(double* grad, double* gradCutoff);
  * uploaded density.  Screening as the reference CPU path (gradient.f90:289-316,
  * shell.f90:1941): quartet and primitive tests both against gradCutoff. */
 void gpu_grad_(double* grad);
-/* gpu.cu:1866 XC on the GPU: not part of this library; records an error */
+/* gpu.cu:1866 B3LYP XC for the uploaded density on the SG-1 grid (isg = 1; the
+ * grid is built on the device once per geometry): o += V_xc, Eelxc += E_xc,
+ * aelec and belec += N/2.  sigrad2: squared radii of significance of the basis
+ * functions (get_sigrad).  The basis arrays it needs (ncontract, itype, aexp,
+ * dcoeff, ncenter) and the atom types come from gpu_upload_basis_ and
+ * gpu_upload_atom_and_chg_. */
 void gpu_getxc_(int* isg, double* sigrad2, double* Eelxc, double* aelec, double* belec,
                 double* o);
 /* gpu.cu:1991 / gpu.cu:1559 NODIRECT (stored-integral) mode: not part of this
@@ -203,6 +208,65 @@ long long qk_scratch_grid_cap(long long lane_bytes, int waves_per_block);
  * row-major.  No device needed. */
 void qk_boys_table_shape(int* rows, int* cols, int* per_unit);
 int qk_boys_table_f32(float* out, int rows, int cols);
+
+/* ---------------- exchange-correlation (B3LYP) ---------------- */
+
+/* The XC step of QUICK's B3LYP operator (dftoperator, dft.f90:894-1018; GPU
+ * getxc_kernel, gpu_getxc.cu:92-267) for a closed-shell density: grid,
+ * Stratmann-Scuseria-Frisch partition, basis values, density, functional and
+ * potential on the device, the two contractions with the density and the
+ * potential on FP64 MFMA, V_xc accumulated in int64 fixed point (bitwise
+ * reproducible). */
+typedef struct qk_xc qk_xc;
+
+enum qk_grid_kind {
+    QK_GRID_SG1 = 1,      /* SG-1: 50 Euler-Maclaurin radial points, pruned Lebedev
+                             6/38/86/194 by region, SG-1 radii (gridformnew) */
+    QK_GRID_PRODUCT = 2   /* nrad Euler-Maclaurin radial points x an nang-point
+                             Lebedev rule (6, 38, 86, 194 or 302), SG-1 radii */
+};
+
+typedef struct qk_xc_opts {
+    int grid;              /* qk_grid_kind */
+    int nrad;              /* QK_GRID_PRODUCT: radial points per atom */
+    int nang;              /* QK_GRID_PRODUCT: Lebedev points per shell */
+    double weight_cut;     /* drop points of smaller weight (QUICK DMCutoff, 1e-10) */
+    double basis_cut;      /* basis-value significance (QUICK basisCufoff, 1e-10) */
+    double density_cut;    /* skip points with rho_alpha below it (QUICK DMCutoff) */
+} qk_xc_opts;
+
+/* SG-1, cutoffs 1e-10 */
+void qk_xc_default_opts(qk_xc_opts* o);
+qk_xc* qk_xc_create(int device);
+void qk_xc_destroy(qk_xc* x);
+/* Atoms (xyz 3 x natom in bohr, atomic numbers) and the Cartesian basis
+ * functions as gpu_upload_basis_ passes them (gpu.cu:1070): ncontract(n),
+ * itype(3, n), aexp(maxcontract, n), dcoeff(maxcontract, n) normalised per
+ * function (normalize_basis), ncenter(n) 1-based.  sigrad2: the squared radii
+ * of significance (get_sigrad, dft.f90:536-613), or NULL to compute them here
+ * with basis_cut. */
+int qk_xc_set_system(qk_xc* x, int natom, const double* xyz, const int* Z, int nbasis, int maxcontract,
+                     const int* ncontract, const int* itype, const double* aexp, const double* dcoeff,
+                     const int* ncenter, const double* sigrad2, double basis_cut);
+/* the squared radii of significance in use (n) */
+int qk_xc_sigrad2(qk_xc* x, double* out);
+/* Build the grid on the device (points, weights, partition, batches). */
+int qk_xc_build_grid(qk_xc* x, const qk_xc_opts* o);
+/* Use a caller's grid instead: npts points as (x, y, z, w) rows, host array;
+ * points outside every function's sphere of significance are dropped. */
+int qk_xc_set_grid(qk_xc* x, long long npts, const double* pts);
+/* points of the grid in use, and the points themselves ((x, y, z, w) rows in
+ * the device's batch order) with their parent atoms (-1 for a caller's grid) */
+long long qk_xc_grid_size(qk_xc* x);
+int qk_xc_get_grid(qk_xc* x, double* pts, int* atom);
+/* batches, mean functions per batch, FP64 MFMA flops of one evaluation */
+int qk_xc_batch_info(qk_xc* x, int* nbatch, double* mean_functions, double* flops);
+/* dV (n x n, device) <- V_xc of the density dD (n x n, device); E_xc and the
+ * electron count (host scalars, may be NULL; when given the call waits for the
+ * stream).  Runs on `stream`. */
+int qk_xc_device(qk_xc* x, const double* dD, double* dV, double* Exc, double* nelec, void* stream);
+/* host arrays, synchronous */
+int qk_xc_host(qk_xc* x, const double* D, double* V, double* Exc, double* nelec);
 
 #ifdef __cplusplus
 }

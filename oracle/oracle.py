@@ -16,10 +16,10 @@ LIB = HERE / "liboracle.so"
 
 
 def build(force: bool = False) -> Path:
-    src = HERE / "qk_oracle.c"
-    if force or not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+    srcs = [HERE / "qk_oracle.c", HERE / "qk_xc_oracle.c"]
+    if force or not LIB.exists() or any(LIB.stat().st_mtime < s.stat().st_mtime for s in srcs):
         subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", str(LIB),
-                        str(src), "-lm"], check=True)
+                        *map(str, srcs), "-lm"], check=True)
     return LIB
 
 
@@ -50,6 +50,13 @@ def lib():
         L.qko_eri.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
         L.qko_grad.restype = C.c_long
         L.qko_grad.argtypes = [P, dp, dp, C.c_double, C.c_double, C.POINTER(C.c_long)]
+        dptr = C.POINTER(C.c_double)
+        L.qkx_b3lyp.argtypes = [C.c_double, C.c_double, dptr, dptr, dptr]
+        L.qkx_sigrad2.argtypes = [C.c_int, ip, dp, ip, C.c_double, dp]
+        L.qkx_sg1_grid.restype = C.c_long
+        L.qkx_sg1_grid.argtypes = [C.c_int, dp, ip, ip, C.POINTER(dptr), C.c_double, dp, C.c_long]
+        L.qkx_xc.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, dp, dp, dp, dp, C.c_long, dp, C.c_double, dp,
+                             dptr, dptr, dptr]
         _lib = L
     return _lib
 
@@ -129,3 +136,65 @@ class Fixture:
         z = np.load(path)
         self.__dict__.update({k: z[k] for k in z.files})
         self.nbasis, self.nshell, self.jbasis = int(self.nbasis), int(self.nshell), int(self.jbasis)
+
+
+# ---------------------------------------------------------------- XC (B3LYP)
+
+def b3lyp(rho: float, sigma: float):
+    """Closed-shell B3LYP f(rho, sigma), df/drho, df/dsigma (qkx_b3lyp)."""
+    f, vr, vs = C.c_double(), C.c_double(), C.c_double()
+    lib().qkx_b3lyp(float(rho), float(sigma), C.byref(f), C.byref(vr), C.byref(vs))
+    return f.value, vr.value, vs.value
+
+
+def _xc_arrays(b):
+    itype = np.ascontiguousarray(np.asarray(b.KLMN, np.int32).T.ravel())   # (l, m, n) per function
+    return (itype, np.ascontiguousarray(b.ncontract, np.int32), np.ascontiguousarray(np.asarray(b.ncenter, np.int32)),
+            _f(b.dcoeff, np.float64), _f(b.aexp, np.float64))
+
+
+def sigrad2(b, cutoff: float = 1e-10) -> np.ndarray:
+    itype, ncon, _, _, aexp = _xc_arrays(b)
+    out = np.zeros(int(b.nbasis))
+    lib().qkx_sigrad2(int(b.nbasis), ncon, aexp, itype, cutoff, out)
+    return out
+
+
+def lebedev_tables(orders=(3, 9, 15, 23)):
+    """Lebedev rules (x, y, z, w; sum w = 4 pi) of 6, 38, 86, 194 points (scipy)."""
+    from scipy.integrate import lebedev_rule
+    out = []
+    for o in orders:
+        x, w = lebedev_rule(o)
+        out.append(np.ascontiguousarray(np.vstack([x, w[None, :]]).T))
+    return out
+
+
+def sg1_grid(xyz_bohr, Z, cut: float = 1e-10) -> np.ndarray:
+    """The SG-1 grid (npts x 5: x, y, z, w, parent atom), qkx_sg1_grid."""
+    xyz = np.ascontiguousarray(np.asarray(xyz_bohr, np.float64).T.ravel())   # (natom, 3)
+    Z = np.ascontiguousarray(Z, np.int32)
+    tabs = lebedev_tables()
+    ns = np.array([len(t) for t in tabs], np.int32)
+    arr = (C.POINTER(C.c_double) * 4)(*[t.ctypes.data_as(C.POINTER(C.c_double)) for t in tabs])
+    cap = len(Z) * 50 * 194
+    out = np.zeros(cap * 5)
+    n = lib().qkx_sg1_grid(len(Z), xyz, Z, ns, arr, cut, out, cap)
+    return out[:5 * n].reshape(n, 5)
+
+
+def xc(b, D, pts, sig2=None, dmcut: float = 1e-10):
+    """B3LYP V_xc (n x n), E_xc, alpha and beta electron counts on the grid
+    pts (npts x >=4: x, y, z, w), restating dftoperator's XC loop."""
+    itype, ncon, ncen, dco, aexp = _xc_arrays(b)
+    n = int(b.nbasis)
+    sig2 = sigrad2(b) if sig2 is None else np.ascontiguousarray(sig2, np.float64)
+    p5 = np.zeros((len(pts), 5))
+    p5[:, :4] = np.asarray(pts)[:, :4]
+    V = np.zeros(n * n)
+    E, a, bb = C.c_double(), C.c_double(), C.c_double()
+    xyz = np.ascontiguousarray(np.asarray(b.xyz, np.float64).T.ravel())
+    lib().qkx_xc(len(b.symbols) if hasattr(b, "symbols") else xyz.size // 3, xyz, n, ncen, itype, ncon, dco, aexp,
+                 sig2, _f(D, np.float64), len(p5), np.ascontiguousarray(p5.ravel()), dmcut, V, C.byref(E),
+                 C.byref(a), C.byref(bb))
+    return V.reshape(n, n, order="F"), E.value, a.value, bb.value

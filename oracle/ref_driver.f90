@@ -22,7 +22,16 @@
 !                + E_nn (hfoperator.f90:791-803, getEnergy.f90:113), and the
 !                occupied orbitals and their energies;
 !   mode "scfgrad": "scf", then at the converged density the full gradient of
-!                hfgrad (gradient.f90:5-327) and its four-centre term alone.
+!                hfgrad (gradient.f90:5-327) and its four-centre term alone;
+!   mode "xc":   (job card B3LYP) the reference's own SG-1 grid -- every point
+!                dftoperator visits, enumerated as dft.f90:906-936 does with
+!                gridformnew, SSW and the radial tables of gridformSG1, with its
+!                weight when that is >= DMCutoff -- and the significance radii
+!                of get_sigrad; then dftoperator for the given density
+!                (dft.f90:618-1553): the final operator (1e + 2e with exchange
+!                weight 0.2 + XC) and Osavedft, the operator before the XC
+!                terms, whose difference is V_xc (lower triangle); alpha and
+!                beta electron counts.  dftoperator prints E_xc ("Eelxc =").
 ! Everything is dumped as raw little-endian binary so the Python fixture writer
 ! can read it without running any reference code itself.
 !
@@ -38,6 +47,9 @@ program qk_ref_driver
    double precision, allocatable :: hold(:,:), hold2(:,:), Fh(:,:,:), Eh(:,:,:), Bm(:,:), cf(:), V2(:,:), Dold(:,:)
    double precision :: Enn, Eel2, Etot, prms, pchg, dist
    integer :: it, nd, k, nocc, ierr2, nsave
+   integer :: Iatm, Irad, Iang, iiangt, npts
+   double precision :: rad, rad3, gx, gy, gz, wpt
+   double precision, external :: ssw
    integer II,JJ,KK,LL,NBI1,NBI2,NBJ1,NBJ2,NBK1,NBK2,NBL1,NBL2
    common /hrrstore/II,JJ,KK,LL,NBI1,NBI2,NBJ1,NBJ2,NBK1,NBK2,NBL1,NBL2
 
@@ -191,6 +203,37 @@ program qk_ref_driver
          call dump_d(trim(odir)//'/grad4c.bin', quick_qm_struct%gradient, 3*natom)
          call dump_d(trim(odir)//'/time_grad.bin', (/ t1-t0 /), 1)
       endif
+   else if (trim(mode) == 'xc') then
+      open(79, file=trim(odir)//'/grid.bin', access='stream', form='unformatted', status='replace')
+      npts = 0
+      do Iatm = 1, natom
+         do Irad = 1, 50
+            call gridformnew(Iatm, RGRID(Irad), iiangt)
+            rad = radii(quick_molspec%iattype(Iatm))
+            rad3 = rad*rad*rad
+            do Iang = 1, iiangt
+               gx = xyz(1,Iatm) + rad*RGRID(Irad)*XANG(Iang)
+               gy = xyz(2,Iatm) + rad*RGRID(Irad)*YANG(Iang)
+               gz = xyz(3,Iatm) + rad*RGRID(Irad)*ZANG(Iang)
+               wpt = ssw(gx, gy, gz, Iatm)*WTANG(Iang)*RWT(Irad)*rad3
+               if (wpt >= quick_method%DMCutoff) then
+                  write(79) gx, gy, gz, wpt, dble(Iatm)
+                  npts = npts + 1
+               endif
+            enddo
+         enddo
+      enddo
+      close(79)
+      call dump_i(trim(odir)//'/npts.bin', (/ npts /), 1)
+      call dump_d(trim(odir)//'/sigrad2.bin', sigrad2, nb)
+      quick_method%printEnergy = .true.
+      call cpu_time(t0)
+      call dftoperator
+      call cpu_time(t1)
+      call dump_d(trim(odir)//'/O.bin', quick_qm_struct%o, nb*nb)
+      call dump_d(trim(odir)//'/Osave.bin', quick_qm_struct%Osavedft, nb*nb)
+      call dump_d(trim(odir)//'/xc_scalars.bin', (/ quick_qm_struct%aelec, quick_qm_struct%belec, &
+                  quick_qm_struct%Eel, t1-t0 /), 4)
    else if (trim(mode) == 'grad') then
       call cpu_time(t0)
       call four_centre()

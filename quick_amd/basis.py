@@ -83,6 +83,13 @@ class Basis:
     gcexpo: np.ndarray      # (MAXPRIM, nbasis)
     nelec: int = 0
     meta: dict = field(default_factory=dict)
+    # per-function contraction of the Cartesian functions the XC step evaluates
+    # (normalize_basis, quick_basis_module_.f90:331-373): each component
+    # normalised on its own, coefficients dcoeff over exponents aexp
+    dcoeff: np.ndarray | None = None   # (MAXPRIM, nbasis)
+    aexp: np.ndarray | None = None     # (MAXPRIM, nbasis)
+    ncontract: np.ndarray | None = None   # (nbasis,)
+    ncenter: np.ndarray | None = None     # (nbasis,) 1-based atom
 
     @property
     def natom(self) -> int:
@@ -122,6 +129,10 @@ def build_basis(symbols, xyz_ang, charge: int = 0, data: dict | None = None) -> 
     cons = np.zeros(nbasis)
     gccoeff = np.zeros((MAXPRIM, nbasis))
     gcexpo = np.zeros((MAXPRIM, nbasis))
+    dcoeff = np.zeros((MAXPRIM, nbasis))
+    aexp = np.zeros((MAXPRIM, nbasis))
+    ncontract = I(nbasis)
+    ncenter = I(nbasis)
 
     jb = 1  # running primitive index (1-based)
     ni = 0  # running function count (Ninitial)
@@ -191,12 +202,24 @@ def build_basis(symbols, xyz_ang, charge: int = 0, data: dict | None = None) -> 
                 gccoeff[:npr, first + q] = [f * v for v in raw[q]]
         else:
             raise ValueError(f"shell type {t} not supported")
+        # the XC step's per-function contraction: raw coefficient x xnorm of the
+        # function's own (l, m, n), renormalised per function
+        sp_first = ksumtype[js] - 1
+        for f in range(sp_first, ni):
+            l, m, n = (int(v) for v in KLMN[:, f])
+            col = 2 if (t == "SP" and f > sp_first) else 1
+            c = [p[col] * xnorm(aa[k], l, m, n) for k, p in enumerate(prims)]
+            fn = xnewnorm(l, m, n, c, aa)
+            dcoeff[:npr, f] = [fn * v for v in c]
+            aexp[:npr, f] = aa
+            ncontract[f] = npr
+            ncenter[f] = atom
         jb += npr
     ksumtype[nshell] = ni + 1
     nelec = int(Z.sum()) - charge
     return Basis(list(symbols), Z, xyz, nshell, nbasis, jbasis, kstart, kprim, katom, ktype,
                  Qnumber, Qstart, Qfinal, Qsbasis, Qfbasis, ksumtype, KLMN, cons, gccoeff,
-                 gcexpo, nelec)
+                 gcexpo, nelec, dcoeff=dcoeff, aexp=aexp, ncontract=ncontract, ncenter=ncenter)
 
 
 def qshells(b: Basis):

@@ -38,7 +38,7 @@ def _compile(src: Path, verbose: bool) -> Path:
     obj = OBJ / (src.stem + ".o")
     deps = [src, CSRC / "qk_kernels.h", CSRC / "qk_internal.h", CSRC / "eri_generated.inc",
             CSRC / "qk_team.h", CSRC / "team_tables.inc", CSRC / "grad_tables.inc", CSRC / "grad_light.inc",
-            CSRC / "qk_grad_light.h",
+            CSRC / "qk_grad_light.h", CSRC / "xc_b3lyp.inc", CSRC / "lebedev.inc",
             HERE.parent / "include" / "quick_mi355x.h", Path(__file__)]
     if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps if d.exists()):
         return obj
@@ -57,9 +57,11 @@ def build(verbose: bool = True, jobs: int = 8) -> Path:
                     str(CSRC / "gen")], check=True, stdout=subprocess.DEVNULL)
     subprocess.run([sys.executable, str(CSRC / "gen_grad.py"), str(CSRC / "grad_tables.inc")], check=True,
                    stdout=subprocess.DEVNULL)
+    subprocess.run([sys.executable, str(CSRC / "gen_xc.py"), str(CSRC / "xc_b3lyp.inc"), str(CSRC / "lebedev.inc")],
+                   check=True, stdout=subprocess.DEVNULL)
     srcs = sorted(list((CSRC / "gen").glob("fock_*.hip")) + list((CSRC / "gen").glob("grad_*.hip")),
                   key=lambda p: -p.stat().st_size)
-    srcs += [CSRC / "qk_fock.hip", CSRC / "qk_grad.hip", CSRC / "qk_host.cpp"]
+    srcs += [CSRC / "qk_xc.hip", CSRC / "qk_fock.hip", CSRC / "qk_grad.hip", CSRC / "qk_host.cpp"]
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -64,6 +64,8 @@ static void set_err(const char* fmt, ...) {
     g_err = buf;
 }
 extern "C" const char* qk_last_error(void) { return g_err.c_str(); }
+// the error text of the other translation units (qk_xc.hip)
+void qk_set_error_text(const char* msg) { g_err = msg; }
 
 #define HIPCHK(call)                                                                      \
     do {                                                                                  \
@@ -1535,14 +1537,31 @@ extern "C" void gpu_upload_atom_and_chg_(int* atom_type, double* chg) {
     g_atom_type.assign(atom_type, atom_type + (size_t)g_natom);
 }
 
-extern "C" void gpu_upload_basis_(int* nshell, int*, int*, int* jbasis, int*, int*, int*, double*, double*, int*,
-                                  int*, int*, int*, int*, int* kstart, int* katom, int* ktype, int* kprim, int*,
-                                  int* Ksumtype, int*, int* Qstart, int* Qfinal, int* Qsbasis, int* Qfbasis,
-                                  double* gccoeff, double* cons, double* gcexpo, int* KLMN) {
+// the per-function arrays the XC step reads (gpu_getxc_), kept from gpu_upload_basis_
+static int g_maxcontract = 0;
+static std::vector<int> g_ncontract, g_itype, g_ncenter;
+static std::vector<double> g_aexp, g_dcoeff;
+static long long g_geometry = 0;        // bumped by every basis upload
+static qk_xc* g_xc = nullptr;
+static long long g_xc_geometry = -1;
+
+extern "C" void gpu_upload_basis_(int* nshell, int*, int*, int* jbasis, int* maxcontract, int* ncontract, int* itype,
+                                  double* aexp, double* dcoeff, int*, int*, int*, int*, int* ncenter, int* kstart,
+                                  int* katom, int* ktype, int* kprim, int*, int* Ksumtype, int*, int* Qstart,
+                                  int* Qfinal, int* Qsbasis, int* Qfbasis, double* gccoeff, double* cons,
+                                  double* gcexpo, int* KLMN) {
     if (!g_ctx) gpu_init_();
     if (qk_set_system(g_ctx, g_natom, g_xyz.data(), *nshell, g_nbasis, *jbasis, kstart, kprim, katom, ktype, Qstart,
                       Qfinal, Qsbasis, Qfbasis, Ksumtype, KLMN, cons, gccoeff, gcexpo))
         fatal("gpu_upload_basis");
+    const size_t n = (size_t)g_nbasis, mc = (size_t)*maxcontract;
+    g_maxcontract = *maxcontract;
+    g_ncontract.assign(ncontract, ncontract + n);
+    g_itype.assign(itype, itype + 3 * n);
+    g_ncenter.assign(ncenter, ncenter + n);
+    g_aexp.assign(aexp, aexp + mc * n);
+    g_dcoeff.assign(dcoeff, dcoeff + mc * n);
+    ++g_geometry;
 }
 extern "C" void gpu_upload_cutoff_matrix_(double* Y, double* cutPrim) {
     if (!g_ctx || qk_set_schwarz(g_ctx, Y, cutPrim)) fatal("gpu_upload_cutoff_matrix");
@@ -1645,8 +1664,50 @@ extern "C" void gpu_grad_(double* grad) {
     abi_consume(c);
     c->abi_have_grad_seed = false;
 }
-extern "C" void gpu_getxc_(int*, double*, double*, double*, double*, double*) {
-    set_err("gpu_getxc: the XC step is not provided by this library");
+// B3LYP XC on the SG-1 grid for the uploaded density (gpu.cu:1866-1975):
+// o += V_xc, Eelxc += E_xc, aelec and belec += N/2; sigrad2 gives the radii of
+// significance of the basis functions (get_sigrad).  The grid is built on the
+// device once per geometry.
+extern "C" void gpu_getxc_(int* isg, double* sigrad2, double* Eelxc, double* aelec, double* belec, double* o) {
+    qk_ctx* c = g_ctx;
+    if (!c || !c->abi_have_dense) {
+        set_err("gpu_getxc: density not uploaded (call gpu_upload_calculated first)");
+        fatal("gpu_getxc");
+    }
+    if (*isg != 1) {
+        set_err("gpu_getxc: only the SG-1 grid (isg = 1) is provided");
+        fatal("gpu_getxc");
+    }
+    if ((int)g_atom_type.size() != g_natom || g_ncontract.empty()) {
+        set_err("gpu_getxc: atoms or basis not uploaded");
+        fatal("gpu_getxc");
+    }
+    if (!g_xc || g_xc_geometry != g_geometry) {
+        if (!g_xc) g_xc = qk_xc_create(c->device);
+        if (!g_xc ||
+            qk_xc_set_system(g_xc, g_natom, g_xyz.data(), g_atom_type.data(), g_nbasis, g_maxcontract,
+                             g_ncontract.data(), g_itype.data(), g_aexp.data(), g_dcoeff.data(), g_ncenter.data(),
+                             sigrad2, 1e-10))
+            fatal("gpu_getxc");
+        qk_xc_opts xo;
+        qk_xc_default_opts(&xo);
+        if (qk_xc_build_grid(g_xc, &xo)) fatal("gpu_getxc");
+        g_xc_geometry = g_geometry;
+    }
+    (void)hipSetDevice(c->device);
+    double E = 0.0, N = 0.0;
+    if (qk_xc_device(g_xc, c->d_D.p, c->d_G.p, &E, &N, nullptr)) fatal("gpu_getxc");
+    const size_t nn = (size_t)c->nbasis * c->nbasis;
+    std::vector<double> V(nn);
+    if (hipMemcpy(V.data(), c->d_G.p, nn * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_err("V_xc download failed");
+        fatal("gpu_getxc");
+    }
+    for (size_t k = 0; k < nn; ++k) o[k] += V[k];
+    *Eelxc += E;
+    *aelec += 0.5 * N;
+    *belec += 0.5 * N;
+    abi_consume(c);
 }
 extern "C" void gpu_aoint_(double*, double*, int*, char*) {
     set_err("gpu_aoint: stored-integral (NODIRECT) mode is not provided by this library");
@@ -1661,6 +1722,9 @@ extern "C" void gpu_cleanup_(void) {
     }
 }
 extern "C" void gpu_shutdown_(void) {
+    qk_xc_destroy(g_xc);
+    g_xc = nullptr;
+    g_xc_geometry = -1;
     qk_destroy(g_ctx);
     g_ctx = nullptr;
     g_started = false;

@@ -6,6 +6,9 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+// sets the text qk_last_error() returns (this thread)
+void qk_set_error_text(const char* msg);
+
 namespace qk {
 
 constexpr int kNumPairClasses = 6;     // ss ps pp ds dp dd (la >= lb)

@@ -18,7 +18,15 @@ orbitals (D = 2 C_occ C_occ^T), their energies, the total energy and its parts,
 H_core, S and G(D) (sampled for systems above 400 functions), the full analytic
 gradient of hfgrad and its four-centre term.
 
-usage (repository root):  python tests/golden/make_golden.py [--scf] [system ...]
+With ``--xc`` it runs the driver's ``xc`` mode (job card B3LYP) at the system's
+converged HF density (from ``<system>_scf.npz``) and stores, as
+``<system>_xc.npz``, the reference's SG-1 grid (points, weights >= 1e-10, parent
+atom), the significance radii, V_xc (dftoperator's operator minus its operator
+before the XC terms), E_xc, the alpha/beta electron counts, and the B3LYP
+two-electron part G(D) with exchange weight 0.2 (the operator before the XC
+terms minus the one-electron operator H of the SCF fixture).
+
+usage (repository root):  python tests/golden/make_golden.py [--scf|--xc] [system ...]
 """
 from __future__ import annotations
 
@@ -48,7 +56,46 @@ SYSTEMS = {
     "water32": (True, False),   # full benchmark size: ~10 min of reference CPU time
     "water64": (True, False),   # 1600 functions: ~1 h of reference CPU time
 }
-SCF_SYSTEMS = ["h2o", "water4", "icosane", "water32"]
+SCF_SYSTEMS = ["h2o", "water2", "water4", "icosane", "water32"]
+XC_SYSTEMS = ["h2o", "water2"]
+
+
+def run_xc(name: str) -> Path:
+    """The reference B3LYP operator (driver mode xc) at the converged HF density."""
+    import re
+    syms, xyz = small_system(name)
+    scf = np.load(Path(__file__).with_name(f"{name}_scf.npz"))
+    C = scf["C_occ"]
+    D = 2.0 * C @ C.T
+    env = dict(os.environ, QUICK_BASIS=BASIS_DIR)
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "job.in").write_text(to_quick_input(syms, xyz, KEYWORDS.replace("HF", "B3LYP")))
+        od = td / "xc"
+        od.mkdir()
+        D.ravel(order="F").tofile(td / "D.bin")
+        r = subprocess.run(f"ulimit -s unlimited && {DRIVER} job.in xc D.bin {od}", shell=True, check=True,
+                           cwd=td, env=env, capture_output=True, text=True)
+        m = re.search(r"Eelxc\s*=\s*([-+0-9.EeDd]+)", r.stdout)
+        out = _read_setup(od, syms)
+        n = int(out["nbasis"])
+        npts = int(np.fromfile(od / "npts.bin", np.int32)[0])
+        g = np.fromfile(od / "grid.bin").reshape(npts, 5)
+        O = np.fromfile(od / "O.bin").reshape(n, n, order="F")
+        Os = np.fromfile(od / "Osave.bin").reshape(n, n, order="F")
+        sc = np.fromfile(od / "xc_scalars.bin")
+        sig2 = np.fromfile(od / "sigrad2.bin")
+    L = np.tril(O - Os)
+    V = L + np.tril(L, -1).T
+    Lg = np.tril(Os - scf["H"])
+    G02 = Lg + np.tril(Lg, -1).T
+    out.update(C_occ=C, grid_xyz=g[:, :3].copy(), grid_w=g[:, 3].copy(), grid_atom=g[:, 4].astype(np.int16),
+               sigrad2=sig2, Vxc=V, Exc=np.array(float(m.group(1).replace("D", "E"))),
+               aelec=np.array(sc[0]), belec=np.array(sc[1]), E_el=np.array(sc[2]), time_dft=np.array(sc[3]),
+               G_b3lyp=G02)
+    dst = Path(__file__).with_name(f"{name}_xc.npz")
+    np.savez_compressed(dst, **out)
+    return dst
 
 
 def _read_setup(od: Path, syms) -> dict:
@@ -154,6 +201,9 @@ def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
             out.pop(k)
         _sample(out, "G", G)
         out.update(E2=np.array(0.5 * np.sum(D * G)), density_seed=np.array(seed))
+        if int(out["nbasis"]) > 1200:
+            Y = out.pop("Ycutoff")
+            _sample(out, "Y", Y, 777)
     dst = Path(__file__).with_name(f"{name}.npz")
     np.savez_compressed(dst, **out)
     return dst
@@ -161,6 +211,13 @@ def run(name: str, fock: bool, grad: bool, seed: int = 7) -> Path:
 
 if __name__ == "__main__":
     args = sys.argv[1:]
+    if args and args[0] == "--xc":
+        for n in args[1:] or XC_SYSTEMS:
+            p = run_xc(n)
+            z = np.load(p)
+            print(p.name, "nbasis", int(z["nbasis"]), "points", len(z["grid_w"]), "Exc", float(z["Exc"]),
+                  "aelec", float(z["aelec"]), f"{p.stat().st_size/1024:.0f} KiB")
+        sys.exit(0)
     if args and args[0] == "--scf":
         for n in args[1:] or SCF_SYSTEMS:
             p = run_scf(n)
