@@ -82,8 +82,10 @@ __device__ double ssf_weight(double gx, double gy, double gz, int ip, const XcAt
             const double mu = (ri - rj) / sqrt(dx * dx + dy * dy + dz * dz);
             if (mu >= 0.64) w = 0.0;
             else if (mu >= -0.64) {
-                const double f = mu / 0.64, f2 = f * f;
-                const double g = f * (35.0 + f2 * (-35.0 + f2 * (21.0 - 5.0 * f2))) / 16.0;
+                // the polynomial in the monomial form of subs/ssw.f90: 1 - g cancels
+                // near |mu| = a, so the form decides the last digits of such weights
+                const double f = mu / 0.64, f3 = f * f * f, f5 = f3 * f * f, f7 = f5 * f * f;
+                const double g = (35.0 * f - 35.0 * f3 + 21.0 * f5 - 5.0 * f7) / 16.0;
                 w *= 0.5 * (1.0 - g);
             }
         }
@@ -354,8 +356,10 @@ __global__ void __launch_bounds__(256) xc_values(const XcBatch* __restrict__ bt,
 // ---- FP64 MFMA tile: C(64 x 64) += sum_k A(i, k) B(k, j), operands staged
 // through LDS in 16 x 64 slabs (k-major).  Block of four wavefronts; wavefront
 // v owns rows 32 (v >> 1) .. +32 and columns 32 (v & 1) .. +32 as 2 x 2 tiles
-// of v_mfma_f64_16x16x4.  Lane l of a 16x16x4 MFMA supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16] and holds C[4 (l / 16) + r][l % 16] in accumulator r.
+// of v_mfma_f64_16x16x4.  Lane l of a 16x16x4 FP64 MFMA supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds C[4 r + l / 16][l % 16] in accumulator r (the
+// FP64 form interleaves the four lane groups by row; the FP32 16x16x4 form
+// instead gives each lane group four consecutive rows).
 __device__ __forceinline__ void mfma_slab(v4d acc[2][2], const double* As, const double* Bs) {
     const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
     const int r0 = 32 * (v >> 1), c0 = 32 * (v & 1);
@@ -417,7 +421,7 @@ __global__ void __launch_bounds__(256) xc_rho_gemm(const int4* __restrict__ task
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = r0 + 16 * i + 4 * (lane >> 4) + r;
+            const int row = r0 + 16 * i + 4 * r + (lane >> 4);
             double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -538,7 +542,7 @@ __global__ void __launch_bounds__(256) xc_v_gemm(const int4* __restrict__ tasks,
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = I0 + r0 + 16 * i + 4 * (lane >> 4) + r, col = J0 + c0 + 16 * j + (lane & 15);
+                const int row = I0 + r0 + 16 * i + 4 * r + (lane >> 4), col = J0 + c0 + 16 * j + (lane & 15);
                 if (row >= B.m || col >= B.m) continue;
                 if (I0 == J0 && row < col) continue;
                 const long long iv = __double2ll_rn(acc[i][j][r] * kXcFix);

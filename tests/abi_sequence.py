@@ -82,7 +82,7 @@ def main(fixture: str, out: str, method: int = 0) -> None:
     L.gpu_get2e_(P(O))
     # gradient.f90:279-287
     seed = rng.normal(size=3 * natom)
-    grad = arr(seed, np.float64)
+    grad = seed.copy()          # gpu_grad_ writes into grad; seed keeps the input
     L.gpu_upload_calculated_(P(O.copy()), P(zd), P(zd), P(dense))
     L.gpu_upload_cutoff_(P(cm), d(1e-11), d(1e-20))
     L.gpu_upload_grad_(P(grad), d(1e-11))

@@ -86,3 +86,35 @@ def test_oracle_planted_mistake_is_seen():
     pts = np.column_stack([z["grid_xyz"], z["grid_w"]])
     _, E, _, _ = O.xc(b, 1.01 * D, pts, z["sigrad2"])
     assert abs(E - float(z["Exc"])) > 1e-3
+
+
+def test_device_functional_matches_oracle(tmp_path):
+    """The generated functional the XC kernels evaluate (xc_b3lyp.inc, from
+    gen_xc.py's symbolic derivatives) compiled for the host with g++, against
+    the oracle's complex-step functional over the density range of real grids."""
+    import ctypes
+    import subprocess
+    import sys
+    from pathlib import Path
+    csrc = Path(__file__).resolve().parents[1] / "quick_amd" / "csrc"
+    sys.path.insert(0, str(csrc))
+    import gen_xc
+    inc = tmp_path / "xc_b3lyp.inc"
+    gen_xc.emit(str(inc))
+    src = tmp_path / "f.cpp"
+    src.write_text('#include <cmath>\nusing std::pow; using std::sqrt; using std::exp; using std::log;\n'
+                   'using std::asinh; using std::atan;\n#define QK_HD\n#include "xc_b3lyp.inc"\n'
+                   'extern "C" void run(double r, double s, double* o) { b3lyp_closed<double>(r, s, o, o + 1, o + 2); }\n')
+    so = tmp_path / "f.so"
+    subprocess.run(["g++", "-O2", "-shared", "-fPIC", "-I", str(tmp_path), str(src), "-o", str(so)], check=True)
+    L = ctypes.CDLL(str(so))
+    out = (ctypes.c_double * 3)()
+    rng = np.random.default_rng(1)
+    for _ in range(400):
+        rho = 10.0 ** rng.uniform(-9, 3)
+        sigma = rho ** (8.0 / 3.0) * 10.0 ** rng.uniform(-4, 3)
+        L.run(ctypes.c_double(rho), ctypes.c_double(sigma), out)
+        f, vr, vs = O.b3lyp(rho, sigma)
+        assert abs(out[0] - f) <= 1e-12 * max(abs(f), 1e-300) + 1e-300
+        assert abs(out[1] - vr) <= 1e-10 * abs(vr)
+        assert abs(out[2] - vs) <= 1e-10 * abs(vs)
