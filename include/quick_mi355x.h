@@ -138,6 +138,19 @@ int qk_prepare(qk_ctx* c, const qk_opts* o);
  * the density dD; both device pointers, n x n doubles.  Runs on `stream`
  * (a hipStream_t, NULL = default) and returns after enqueueing. */
 int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): qk_fock_device with its per-build scratch
+ * (one int64 accumulator, counters, scaled density, shell-block density maxima)
+ * in a fixed device array of 48 MiB that each GPU holds until the process ends,
+ * and no hipMalloc, hipFree or allocating library call.  dG is bitwise the
+ * operator qk_fock_device writes.  Requires the work lists for the same options
+ * on the device (qk_prepare, or an earlier build): -6 otherwise; -7 when the
+ * system needs more scratch than the array holds (with 0.4 shells per function,
+ * as 6-31G** water: 1946 functions in FP64, 1706 in mixed precision).  Calls that overlap on one GPU (other streams, other
+ * contexts) share the array and race.  The per-class statistics and the range
+ * flag of this build are kept in the array, not reported. */
+int qk_fock_device_noalloc(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream);
 /* Host convenience: F <- F + G(D) (host arrays), synchronous */
 int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 

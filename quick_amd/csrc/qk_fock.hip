@@ -21,6 +21,8 @@
 // classes use the wavefront-cooperative kernel (qk_team.h) instead.
 #include "qk_kernels.h"
 
+#include <mutex>
+
 namespace qk {
 
 // ---------------------------------------------------------------------------
@@ -238,6 +240,44 @@ void launch_fock_finalize(const unsigned long long* X, int copies, const double*
                           unsigned long long* flag, hipStream_t s) {
     hipLaunchKernelGGL(fock_finalize, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s, X, copies, cons, G, n,
                        flag);
+}
+
+// ---------------------------------------------------------------------------
+// the no-allocation entry point's scratch
+
+// The device array itself.  In host code `scratch` names a host-side stand-in,
+// not device memory: host code uses only the address noalloc_scratch_init
+// looked up for the device (kernel arguments, fills, copies).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+int g_scratch_rc[kMaxScratchDevices] = {};
+bool g_scratch_done[kMaxScratchDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_done[device]) return g_scratch_rc[device];
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    g_scratch_done[device] = true;
+    g_scratch_rc[device] = (int)e;
+    g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+    return (int)e;
+}
+
+void* noalloc_scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace qk

@@ -365,6 +365,9 @@ extern "C" qk_ctx* qk_create(int device) {
     }
     qk_ctx* c = new qk_ctx();
     c->device = device;
+    // the no-allocation entry point's scratch address on this device (a failure
+    // is kept and reported by that entry point; the rest of the library works)
+    (void)noalloc_scratch_init(device);
     for (auto& s : c->streams)
         if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
             set_err("stream creation failed");
@@ -778,20 +781,32 @@ extern "C" int qk_prepare(qk_ctx* c, const qk_opts* o) {
 // ---------------------------------------------------------------------------
 // Fock build
 
-static FockArgs make_args(qk_ctx* c, const qk_opts* o) {
+// the per-build scratch of a Fock build: the context's buffers, or the fixed
+// scratch array of the no-allocation entry point
+struct FockBufs {
+    unsigned long long* X;          // xcopies packed triangles
+    int xcopies;
+    unsigned long long* counters;   // kNumCounters
+    double* Dp;                     // n x n
+    float* Dpf;                     // n x n (mixed precision)
+    double* cm;                     // nshell x nshell
+    bool compact;                   // density compaction into the prepared ket lists
+};
+
+static FockArgs make_args(qk_ctx* c, const qk_opts* o, const FockBufs& fb) {
     FockArgs a;
     a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
     a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
     a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
     a.ppf = {c->dkf_zeta.p, c->dkf_oo2z.p, c->dkf_Px.p, c->dkf_Py.p, c->dkf_Pz.p, c->dkf_K.p, c->dkf_cut.p};
-    a.Dpf = c->d_Dpf.p;
+    a.Dpf = fb.Dpf;
     a.boys_tab = c->d_boys_tab.p;
     a.xyz = c->d_xyz.p;
-    a.Dp = c->d_Dp.p;
-    a.cutmat = c->d_cm.p;
-    a.X = c->d_X.p;
-    a.counters = c->d_counters.p;
-    a.xcopies = kXCopies;
+    a.Dp = fb.Dp;
+    a.cutmat = fb.cm;
+    a.X = fb.X;
+    a.counters = fb.counters;
+    a.xcopies = fb.xcopies;
     a.n = c->nbasis;
     a.nshell = c->nshell;
     a.tau = o->tau;
@@ -832,20 +847,27 @@ static int fold_timing(qk_ctx* c) {
 }
 
 // enqueue the whole build on `s` (classes fan out to the context's streams and join back)
+// (fb: the scratch to use; NULL = the context's own, preparing the work lists
+// for these options first)
 static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_null, double* dG, const qk_opts* o,
-                        hipStream_t s) {
-    if (prepare(c, o)) return -1;
+                        hipStream_t s, const FockBufs* fb = nullptr) {
+    FockBufs own;
+    if (!fb) {
+        if (prepare(c, o)) return -1;
+        own = {c->d_X.p, kXCopies, c->d_counters.p, c->d_Dp.p, c->d_Dpf.p, c->d_cm.p, true};
+        fb = &own;
+    }
     const int n = c->nbasis, ns = c->nshell;
     const bool mixed = o->precision == QK_MIXED;
-    launch_scale_density(dD, c->d_cons.p, c->d_Dp.p, mixed ? c->d_Dpf.p : nullptr, n, s);
+    launch_scale_density(dD, c->d_cons.p, fb->Dp, mixed ? fb->Dpf : nullptr, n, s);
     if (dcutmat_or_null)
-        HIPCHK(hipMemcpyAsync(c->d_cm.p, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(fb->cm, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
-        launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, c->d_cm.p, n, ns, s);
-    launch_fill_u64(c->d_X.p, xtri_size(n) * kXCopies, s);
-    launch_fill_u64(c->d_counters.p, kNumCounters, s);
+        launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, fb->cm, n, ns, s);
+    launch_fill_u64(fb->X, xtri_size(n) * fb->xcopies, s);
+    launch_fill_u64(fb->counters, kNumCounters, s);
     HIPCHK(hipGetLastError());
-    FockArgs a = make_args(c, o);
+    FockArgs a = make_args(c, o, *fb);
     const CompactArgs ca = compact_args(c, a, 0, a.tau);
     if (fold_timing(c)) return -1;
     // heaviest classes first, round-robin over the streams
@@ -873,7 +895,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         const size_t lds32 = use_runs ? rows_lds + slab32 : 4 * slab32;
         if (!CH.tiles.empty()) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
-            a.counters = c->d_counters.p + 2 * k;
+            a.counters = fb->counters + 2 * k;
             if (fock_class_is_team(k)) {
                 ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
                 // one 64-lane block per tile; up to 16 resident per CU
@@ -881,7 +903,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 launch_team_class(k, a, w, grid, s);
             } else {
                 const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch(k), 4));
-                if (use_runs && CH.compact) {
+                if (use_runs && CH.compact && fb->compact) {
                     // density screen first: every lane of the class kernel then has a quartet
                     launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
@@ -901,7 +923,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[k] = c->timing;
         }
         if (mixed && !CH.tiles32.empty()) {
-            a.counters = c->d_counters.p + 2 * kNumQuartetClasses + 2 * k;
+            a.counters = fb->counters + 2 * kNumQuartetClasses + 2 * k;
             const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch_f32(k), 4));
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
@@ -910,7 +932,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             size_t lds = lds32;
             // one 64-lane block per tile; up to 20 resident per CU (five wavefronts per SIMD)
             if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 20);
-            if (use_runs && CH.compact32) {
+            if (use_runs && CH.compact32 && fb->compact) {
                 launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
                 w.runs = CH.d_segruns32.p;
                 w.nruns = (long long)CH.segruns32.size();
@@ -932,7 +954,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[slot] = c->timing;
         }
     }
-    launch_fock_finalize(c->d_X.p, kXCopies, c->d_cons.p, dG, n, c->d_counters.p + kRangeFlag, s);
+    launch_fock_finalize(fb->X, fb->xcopies, c->d_cons.p, dG, n, fb->counters + kRangeFlag, s);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -943,6 +965,54 @@ extern "C" int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_
     if (!o) { qk_default_opts(&d); o = &d; }
     (void)hipSetDevice(c->device);
     return fock_enqueue(c, dD, nullptr, dG, o, (hipStream_t)stream);
+}
+
+// true when the work lists on the device were built for these options
+static bool prepared_for(const qk_ctx* c, const qk_opts* o) {
+    const bool mixed = o->precision == QK_MIXED;
+    return c->prepared && c->prep_tau == o->tau && c->prep_pair_cut == o->pair_cut &&
+           c->prep_precision == o->precision && (!mixed || c->prep_fp32_thr == o->fp32_threshold);
+}
+
+// The Fock build with its per-build scratch in the fixed device array: one
+// accumulator copy, the plain tile runs instead of the density compaction (the
+// same quartets contribute the same fixed-point values, so the operator is
+// bitwise that of qk_fock_device), no hipMalloc / hipFree / allocating library
+// call.  The work lists must already be on the device (qk_prepare, or an
+// earlier build with the same options).
+extern "C" int qk_fock_device_noalloc(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream) {
+    if (!c) { set_err("null context"); return -1; }
+    qk_opts d;
+    if (!o) { qk_default_opts(&d); o = &d; }
+    (void)hipSetDevice(c->device);
+    if (o->precision != QK_FP64 && o->precision != QK_MIXED) {
+        set_err("precision must be QK_FP64 or QK_MIXED (got %d)", o->precision);
+        return -4;
+    }
+    if (!prepared_for(c, o)) {
+        set_err("qk_fock_device_noalloc: no work lists for these options on the device (call qk_prepare first)");
+        return -6;
+    }
+    unsigned char* base = (unsigned char*)noalloc_scratch_ptr(c->device);
+    if (!base) {
+        set_err("qk_fock_device_noalloc: the scratch array's address on device %d is unknown (hip error %d)",
+                c->device, noalloc_scratch_init(c->device));
+        return -1;
+    }
+    const bool mixed = o->precision == QK_MIXED;
+    const long long n = c->nbasis, ns = c->nshell;
+    auto up = [](long long b) { return (b + 255) / 256 * 256; };
+    const long long oX = 0, oC = oX + up(8 * xtri_size((int)n)), oD = oC + up(8LL * kNumCounters),
+                    oDf = oD + up(8 * n * n), oCm = oDf + (mixed ? up(4 * n * n) : 0), end = oCm + up(8 * ns * ns);
+    if (end > kNoAllocScratchBytes) {
+        set_err("qk_fock_device_noalloc: %lld functions and %lld shells need %lld bytes of scratch, more than the "
+                "%lld of the fixed array",
+                n, ns, end, (long long)kNoAllocScratchBytes);
+        return -7;
+    }
+    const FockBufs fb{(unsigned long long*)(base + oX), 1, (unsigned long long*)(base + oC), (double*)(base + oD),
+                      mixed ? (float*)(base + oDf) : nullptr, (double*)(base + oCm), false};
+    return fock_enqueue(c, dD, nullptr, dG, o, (hipStream_t)stream, &fb);
 }
 
 // Algorithmic operation count of one class (FMA = 2): per primitive quartet the

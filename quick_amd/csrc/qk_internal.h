@@ -56,6 +56,20 @@ inline long long kernel_scratch(const void* k) {
     return hipFuncGetAttributes(&at, k) == hipSuccess ? (long long)at.localSizeBytes : 0;
 }
 
+// Scratch of the no-allocation Fock entry point (qk_fock_device_noalloc): one
+// fixed device array of this many bytes, defined in qk_fock.hip; every GPU the
+// library runs on holds its own copy until the process ends.  It carries one
+// accumulator copy (n(n+1)/2 int64), the counters, the scaled density (n^2
+// doubles, plus n^2 floats in mixed precision) and the shell-block density
+// maxima (nshell^2 doubles): 44 MB at 1600 functions in mixed precision.
+constexpr long long kNoAllocScratchBytes = 48LL << 20;
+// Looks the array's address up on `device` (hipGetSymbolAddress, with that
+// device current) the first time it is asked for that device; returns its
+// HIP error code (0 on success; later calls return the first call's code).
+int noalloc_scratch_init(int device);
+// the array's address on `device`, nullptr before a successful init
+void* noalloc_scratch_ptr(int device);
+
 constexpr int kFixedShift = 48;         // Fock accumulator: int64 in units of 2^-48
 constexpr int kGradShift = 44;          // gradient accumulator: int64 in units of 2^-44
 

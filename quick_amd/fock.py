@@ -66,6 +66,7 @@ def lib() -> C.CDLL:
         L.qk_compute_schwarz.argtypes = [P, dp, dp]
         L.qk_prepare.argtypes = [P, C.POINTER(QkOpts)]
         L.qk_fock_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
+        L.qk_fock_device_noalloc.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_fock_host.argtypes = [P, dp, dp, C.POINTER(QkOpts)]
         L.qk_grad_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_grad_host.argtypes = [P, dp, dp, C.POINTER(QkOpts)]
@@ -168,6 +169,14 @@ class FockEngine:
         opts = opts or make_opts()
         _check(lib().qk_fock_device(self.h, C.c_void_p(D_ptr), C.c_void_p(G_ptr), C.byref(opts),
                                     C.c_void_p(stream)), "qk_fock_device")
+
+    def fock_device_noalloc(self, D_ptr: int, G_ptr: int, opts: QkOpts | None = None, stream: int = 0) -> int:
+        """qk_fock_device with the per-build scratch in the library's fixed device
+        array (no allocation); the work lists must be prepared for ``opts``.
+        Returns the entry point's code (0, or -6 / -7 with last_error())."""
+        opts = opts or make_opts()
+        return int(lib().qk_fock_device_noalloc(self.h, C.c_void_p(D_ptr), C.c_void_p(G_ptr), C.byref(opts),
+                                                C.c_void_p(stream)))
 
     def fock(self, D, opts: QkOpts | None = None) -> np.ndarray:
         """Host arrays: returns G(D) (n x n)."""
