@@ -108,8 +108,7 @@ __global__ void fill_u64(unsigned long long* p, long long count) {
     if (k < count) p[k] = 0ull;
 }
 
-// G = diag(cons) (X + X^T)/2 diag(cons) in FP64.  The sum T of the copies'
-// slot {i, j} holds X[i][j] + X[j][i] for i != j and X[i][i] on the diagonal.
+// G = diag(cons) (X + X^T)/2 diag(cons) in FP64, X the sum of the copies.
 // A sum within a factor of two of the int64 range (|T| >= 2^62, i.e. |G| of
 // about 8192 and more) raises the range flag: the accumulator would wrap at
 // twice that.
@@ -118,12 +117,32 @@ __global__ void fock_finalize(const unsigned long long* __restrict__ X, int copi
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long long)n * n) return;
     const int i = int(k / n), j = int(k % n);
-    const long long t = xtri(i, j), ts = xtri_size(n);
+    const long long nn = (long long)n * n, t = (long long)i * n + j, tt = (long long)j * n + i;
     unsigned long long u = 0;
-    for (int c = 0; c < copies; ++c) u += X[c * ts + t];
+    for (int c = 0; c < copies; ++c) u += X[c * nn + t] + X[c * nn + tt];
     const long long s = (long long)u;
     if (s >= (1LL << 62) || s <= -(1LL << 62)) atomicOr(flag, 1ull);
-    G[k] = cons[i] * cons[j] * ((double)s * (i == j ? 0x1p-48 : 0x1p-49));
+    G[k] = cons[i] * cons[j] * ((double)s * 0x1p-49);
+}
+
+// The pair-indexed Coulomb buffer into X (copy 0): one thread per significant
+// pair k, element (ia, ib) of its block at J[jofs[k] + (ia NB + ib) pstride[k]]
+// to X[f0(qa) + ia][f0(qb) + ib].  Each function pair belongs to one Qshell
+// pair, so the adds of different threads never meet.
+__global__ void jpair_scatter(const unsigned long long* __restrict__ J, const int* __restrict__ jofs,
+                              const int* __restrict__ pstride, const int* __restrict__ qa,
+                              const int* __restrict__ qb, const DevQshells q, int npair,
+                              unsigned long long* __restrict__ X, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npair) return;
+    const int a = qa[k], b = qb[k];
+    const int fa = q.f0[a], fb = q.f0[b], na = q.nc[a], nb = q.nc[b];
+    const long long o = jofs[k], st = pstride[k];
+    for (int ia = 0; ia < na; ++ia)
+        for (int ib = 0; ib < nb; ++ib) {
+            const unsigned long long v = J[o + (ia * nb + ib) * st];
+            if (v) atomicAdd(X + (long long)(fa + ia) * n + fb + ib, v);
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -235,6 +254,13 @@ void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int
 }
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
+}
+void launch_jpair_scatter(const unsigned long long* J, const int* jofs, const int* pstride, const int* qa,
+                          const int* qb, const DevQshells& q, int npair, unsigned long long* X, int n,
+                          hipStream_t s) {
+    if (npair > 0)
+        hipLaunchKernelGGL(jpair_scatter, dim3(blocks_for(npair, 256)), dim3(256), 0, s, J, jofs, pstride, qa, qb, q,
+                           npair, X, n);
 }
 void launch_fock_finalize(const unsigned long long* X, int copies, const double* cons, double* G, int n,
                           unsigned long long* flag, hipStream_t s) {

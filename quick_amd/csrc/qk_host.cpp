@@ -42,6 +42,9 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
 void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
+void launch_jpair_scatter(const unsigned long long* J, const int* jofs, const int* pstride, const int* qa,
+                          const int* qb, const DevQshells& q, int npair, unsigned long long* X, int n,
+                          hipStream_t s);
 void launch_fock_finalize(const unsigned long long* X, int copies, const double* cons, double* G, int n,
                           unsigned long long* flag, hipStream_t s);
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
@@ -295,7 +298,7 @@ struct qk_ctx {
 
     // device data
     DBuf<double> d_xyz, d_cons;
-    DBuf<int> dq_atom, dq_f0, dq_shell, dq_qf;
+    DBuf<int> dq_atom, dq_f0, dq_shell, dq_qf, dq_nc;
     DBuf<int> dp_qa, dp_qb, dp_prim0, dp_nprim;
     DBuf<int> dp_sh, dp_at;   // per pair: shells and atoms of its two Qshells, 16 bits each (< 2^15)
     DBuf<double> dp_Y;
@@ -303,6 +306,10 @@ struct qk_ctx {
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
+    // pair-indexed Coulomb buffer of the prepared pair list (FockArgs::J)
+    DBuf<int> dp_jofs;
+    DBuf<unsigned long long> d_J;
+    long long jsize = 0;
 
     hipStream_t streams[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev_start = nullptr, ev_cls[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -464,6 +471,10 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         set_err("invalid sizes natom=%d nshell=%d nbasis=%d jbasis=%d", natom, nshell, nbasis, jbasis);
         return -2;
     }
+    if (nbasis > kMaxBasis) {
+        set_err("%d basis functions: at most %d are supported", nbasis, kMaxBasis);
+        return -2;
+    }
     c->natom = natom; c->nshell = nshell; c->nbasis = nbasis; c->jbasis = jbasis;
     c->xyz.assign(xyz, xyz + 3 * (size_t)natom);
     c->kstart.assign(kstart, kstart + nshell);
@@ -531,8 +542,13 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         c->dq_f0.upload(c->q_f0) || c->dq_shell.upload(c->q_shell) || c->dq_qf.upload(c->q_qf) ||
         c->d_sh_f0.upload(sh_f0) || c->d_sh_nf.upload(sh_nf))
         return -1;
+    {
+        std::vector<int> qnc(c->q_L.size());
+        for (size_t k = 0; k < qnc.size(); ++k) qnc[k] = ncart_of(c->q_L[k]);
+        if (c->dq_nc.upload(qnc)) return -1;
+    }
     size_t nn = (size_t)nbasis * nbasis;
-    if (c->d_Dp.alloc(nn) || c->d_X.alloc((size_t)xtri_size(nbasis) * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
+    if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
     if (build_all_pairs(c)) return -1;
     c->have_system = true;
     c->have_schwarz = false;
@@ -632,6 +648,15 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
             pstride[k] = g.count;
         }
     }
+    // the pair-indexed Coulomb buffer: per group, element-major blocks
+    std::vector<int> jofs(npair);
+    long long jsize = 0;
+    for (const Group& g : groups) {
+        const int ne = ncart_of(kPairL[g.cls][0]) * ncart_of(kPairL[g.cls][1]);
+        for (int k = g.start; k < g.start + g.count; ++k) jofs[k] = (int)(jsize + (k - g.start));
+        jsize += (long long)ne * g.count;
+        if (jsize > INT32_MAX) { set_err("pair Coulomb buffer beyond 2^31 entries"); return -1; }
+    }
 
     // per-class tiles: for every (bra group, ket group) block and every bra,
     // the prefix of kets passing Y_ij*Y_kl > tau cut into runs of <= 64
@@ -718,8 +743,9 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dk_zeta.upload(kz) || c->dk_oo2z.upload(ko) ||
         c->dk_Px.upload(kx) || c->dk_Py.upload(ky) || c->dk_Pz.upload(kzz) || c->dk_K.upload(kK) ||
         c->dk_cut.upload(kc) || c->dk_ea.upload(kea) || c->dk_eb.upload(keb) || c->dp_sh.upload(psh) ||
-        c->dp_at.upload(pat))
+        c->dp_at.upload(pat) || c->dp_jofs.upload(jofs) || c->d_J.alloc((size_t)std::max(jsize, 1LL)))
         return -1;
+    c->jsize = jsize;
     c->grp_cls.clear(); c->grp_start.clear(); c->grp_count.clear();
     for (const Group& g : groups) {
         c->grp_cls.push_back(g.cls);
@@ -787,6 +813,7 @@ struct FockBufs {
     unsigned long long* X;          // xcopies packed triangles
     int xcopies;
     unsigned long long* counters;   // kNumCounters
+    unsigned long long* J;          // pair-indexed Coulomb buffer (jsize)
     double* Dp;                     // n x n
     float* Dpf;                     // n x n (mixed precision)
     double* cm;                     // nshell x nshell
@@ -795,7 +822,7 @@ struct FockBufs {
 
 static FockArgs make_args(qk_ctx* c, const qk_opts* o, const FockBufs& fb) {
     FockArgs a;
-    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p, c->dq_nc.p};
     a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
     a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
     a.ppf = {c->dkf_zeta.p, c->dkf_oo2z.p, c->dkf_Px.p, c->dkf_Py.p, c->dkf_Pz.p, c->dkf_K.p, c->dkf_cut.p};
@@ -806,6 +833,8 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o, const FockBufs& fb) {
     a.cutmat = fb.cm;
     a.X = fb.X;
     a.counters = fb.counters;
+    a.J = fb.J;
+    a.jofs = c->dp_jofs.p;
     a.xcopies = fb.xcopies;
     a.n = c->nbasis;
     a.nshell = c->nshell;
@@ -854,7 +883,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     FockBufs own;
     if (!fb) {
         if (prepare(c, o)) return -1;
-        own = {c->d_X.p, kXCopies, c->d_counters.p, c->d_Dp.p, c->d_Dpf.p, c->d_cm.p, true};
+        own = {c->d_X.p, kXCopies, c->d_counters.p, c->d_J.p, c->d_Dp.p, c->d_Dpf.p, c->d_cm.p, true};
         fb = &own;
     }
     const int n = c->nbasis, ns = c->nshell;
@@ -864,7 +893,8 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         HIPCHK(hipMemcpyAsync(fb->cm, dcutmat_or_null, sizeof(double) * ns * ns, hipMemcpyDeviceToDevice, s));
     else
         launch_density_cutoff(dD, c->d_sh_f0.p, c->d_sh_nf.p, fb->cm, n, ns, s);
-    launch_fill_u64(fb->X, xtri_size(n) * fb->xcopies, s);
+    launch_fill_u64(fb->X, (long long)n * n * fb->xcopies, s);
+    launch_fill_u64(fb->J, c->jsize, s);
     launch_fill_u64(fb->counters, kNumCounters, s);
     HIPCHK(hipGetLastError());
     FockArgs a = make_args(c, o, *fb);
@@ -954,6 +984,8 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[slot] = c->timing;
         }
     }
+    launch_jpair_scatter(fb->J, c->dp_jofs.p, c->dp_pstride.p, c->dp_qa.p, c->dp_qb.p, a.q, c->nclass_pairs, fb->X,
+                         n, s);
     launch_fock_finalize(fb->X, fb->xcopies, c->d_cons.p, dG, n, fb->counters + kRangeFlag, s);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1002,16 +1034,18 @@ extern "C" int qk_fock_device_noalloc(qk_ctx* c, const double* dD, double* dG, c
     const bool mixed = o->precision == QK_MIXED;
     const long long n = c->nbasis, ns = c->nshell;
     auto up = [](long long b) { return (b + 255) / 256 * 256; };
-    const long long oX = 0, oC = oX + up(8 * xtri_size((int)n)), oD = oC + up(8LL * kNumCounters),
-                    oDf = oD + up(8 * n * n), oCm = oDf + (mixed ? up(4 * n * n) : 0), end = oCm + up(8 * ns * ns);
+    const long long oX = 0, oJ = oX + up(8 * n * n), oC = oJ + up(8 * std::max(c->jsize, 1LL)),
+                    oD = oC + up(8LL * kNumCounters), oDf = oD + up(8 * n * n),
+                    oCm = oDf + (mixed ? up(4 * n * n) : 0), end = oCm + up(8 * ns * ns);
     if (end > kNoAllocScratchBytes) {
         set_err("qk_fock_device_noalloc: %lld functions and %lld shells need %lld bytes of scratch, more than the "
                 "%lld of the fixed array",
                 n, ns, end, (long long)kNoAllocScratchBytes);
         return -7;
     }
-    const FockBufs fb{(unsigned long long*)(base + oX), 1, (unsigned long long*)(base + oC), (double*)(base + oD),
-                      mixed ? (float*)(base + oDf) : nullptr, (double*)(base + oCm), false};
+    const FockBufs fb{(unsigned long long*)(base + oX), 1, (unsigned long long*)(base + oC),
+                      (unsigned long long*)(base + oJ), (double*)(base + oD), mixed ? (float*)(base + oDf) : nullptr,
+                      (double*)(base + oCm), false};
     return fock_enqueue(c, dD, nullptr, dG, o, (hipStream_t)stream, &fb);
 }
 
@@ -1205,7 +1239,7 @@ extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_o
     }
     start[kNumPairClasses] = (int)list.size();
     if (d_list.upload(list)) return -1;
-    DevQshells qs{c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    DevQshells qs{c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p, c->dq_nc.p};
     DevPrims pp{c->dpp_zeta.p, c->dpp_oo2z.p, c->dpp_Px.p, c->dpp_Py.p, c->dpp_Pz.p, c->dpp_K.p, c->dpp_cut.p};
     for (int cls = 0; cls < kNumPairClasses; ++cls) {
         launch_schwarz(cls, d_list.p + start[cls], start[cls + 1] - start[cls], c->dap_qa.p, c->dap_qb.p,
@@ -1397,7 +1431,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     launch_fill_u64(c->d_gcounters.p, kNumGradCounters, s);
     HIPCHK(hipGetLastError());
     GradArgs a;
-    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p};
+    a.q = {c->dq_atom.p, c->dq_f0.p, c->dq_shell.p, c->dq_qf.p, c->dq_nc.p};
     a.pr = {c->dp_qa.p, c->dp_qb.p, c->dp_prim0.p, c->dp_pstride.p, c->dp_nprim.p, c->dp_Y.p};
     a.pp = {c->dk_zeta.p, c->dk_oo2z.p, c->dk_Px.p, c->dk_Py.p, c->dk_Pz.p, c->dk_K.p, c->dk_cut.p};
     a.ea = c->dk_ea.p;

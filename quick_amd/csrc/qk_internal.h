@@ -23,19 +23,22 @@ constexpr int kRangeFlag = kNumCounters - 1;
 // then the gradient accumulator's range flag
 constexpr int kNumGradCounters = 4 * kNumQuartetClasses + 1;
 constexpr int kGradRangeFlag = kNumGradCounters - 1;
-// The operator accumulator X holds the unordered pairs {i, j} of basis
-// functions as a packed lower triangle (n(n+1)/2 int64, see xtri): G depends on
-// X + X^T only, so a contribution to X[i][j] or X[j][i] lands in one slot.  It
-// is kept in up to kXCopies copies; block b adds into copy b % copies, and the
+// The operator accumulator X is n x n int64 (G = (X + X^T)/2): the exchange
+// rows of a bra are added to X once per run as whole rows, which a full matrix
+// keeps contiguous (in a packed triangle half of each row is strided by the
+// column, and the heaviest thread-per-quartet class ran twice as long).  It is
+// kept in up to kXCopies copies; block b adds into copy b % copies, and the
 // copies are summed at the end.  Consecutive blocks run on different XCDs, so
-// each copy's atomics come mostly from one XCD (at 800 functions a copy is
-// 2.6 MB, within one XCD's 4 MB L2).
+// each copy's atomics come mostly from one XCD.
 constexpr int kXCopies = 8;
 __host__ __device__ inline long long xtri_size(int n) { return (long long)n * (n + 1) / 2; }
-// slot of the unordered pair {i, j} in the packed lower triangle
-__host__ __device__ __forceinline__ long long xtri(int i, int j) {
-    const int hi = i > j ? i : j, lo = i > j ? j : i;
-    return (long long)hi * (hi + 1) / 2 + lo;
+// slot of the unordered pair {i, j} in the packed lower triangle, in 32-bit
+// unsigned arithmetic (exact for n <= kMaxBasis: hi (hi + 1) < 2^32); the
+// 64-bit form doubled the spills of the heaviest thread-per-quartet class
+constexpr int kMaxBasis = 65535;
+__host__ __device__ __forceinline__ unsigned xtri(int i, int j) {
+    const unsigned hi = (unsigned)(i > j ? i : j), lo = (unsigned)(i > j ? j : i);
+    return ((hi * (hi + 1u)) >> 1) + lo;
 }
 // Scratch (private segment) backing of a dispatch: the runtime sizes it as the
 // kernel's per-lane scratch x 64 lanes x the waves the grid can have in flight
@@ -59,10 +62,11 @@ inline long long kernel_scratch(const void* k) {
 // Scratch of the no-allocation Fock entry point (qk_fock_device_noalloc): one
 // fixed device array of this many bytes, defined in qk_fock.hip; every GPU the
 // library runs on holds its own copy until the process ends.  It carries one
-// accumulator copy (n(n+1)/2 int64), the counters, the scaled density (n^2
-// doubles, plus n^2 floats in mixed precision) and the shell-block density
-// maxima (nshell^2 doubles): 44 MB at 1600 functions in mixed precision.
-constexpr long long kNoAllocScratchBytes = 48LL << 20;
+// accumulator copy (n^2 int64), the pair-indexed Coulomb buffer (at most
+// n(n+1)/2 int64), the counters, the scaled density (n^2 doubles, plus n^2
+// floats in mixed precision) and the shell-block density maxima (nshell^2
+// doubles): 62 MB at 1600 functions in mixed precision.
+constexpr long long kNoAllocScratchBytes = 64LL << 20;
 // Looks the array's address up on `device` (hipGetSymbolAddress, with that
 // device current) the first time it is asked for that device; returns its
 // HIP error code (0 on success; later calls return the first call's code).
@@ -79,6 +83,7 @@ struct DevQshells {
     int* f0;      // first basis function (0-based)
     int* shell;   // parent shell (0-based)
     int* qfinal;  // Qfinal of the parent shell (QUICK's Boys-order convention)
+    int* nc;      // Cartesian functions of the Qshell (1, 3, 6)
 };
 
 // Per significant Qshell pair (oriented: L(a) >= L(b); equal L: a <= b).
@@ -224,8 +229,14 @@ struct FockArgs {
     const double* xyz;        // 3 x natom
     const double* Dp;         // cons-scaled density, n x n
     const double* cutmat;     // nshell x nshell, max |D| over shell blocks
-    unsigned long long* X;    // fixed-point accumulator: xcopies packed triangles (xtri)
+    unsigned long long* X;    // fixed-point accumulator: xcopies n x n copies
     unsigned long long* counters;  // [0] surviving quartets, [1] primitive quartets
+    // Coulomb contributions per significant pair: element e of pair k's block
+    // (e over its NA x NB functions) at J[jofs[k] + e * pstride[k]], so that the
+    // lanes of a wavefront, which hold consecutive ket pairs of one group, add
+    // to consecutive addresses; jpair_scatter adds J into X after the classes
+    unsigned long long* J;
+    const int* jofs;
     int xcopies;              // copies of the accumulator (1 .. kXCopies)
     int n;
     int nshell;

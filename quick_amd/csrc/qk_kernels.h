@@ -177,7 +177,7 @@ __device__ __forceinline__ void xk_add(unsigned long long* X, unsigned long long
     const long long iv = __double2ll_rn(v * 0x1p48);
     if (iv == 0) return;
     if (xr) atomicAdd(xr + (long long)r * n + col, (unsigned long long)iv);
-    else atomicAdd(X + xtri((int)grow, col), (unsigned long long)iv);
+    else atomicAdd(X + grow * n + col, (unsigned long long)iv);
 }
 
 __device__ __forceinline__ void fx_add(unsigned long long* X, long long idx, double v) {
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const double X0 = 34.98683665524972;  // 2 pi^(5/2), quick_constants_module.f90:55
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * xtri_size(n);
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * n * n;
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                     double s = 0.0;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
-                    fx_add(Xc, xtri(fc + ic, fd + id), jw * s);
+                    fx_add(a.J, a.jofs[q] + (long long)(ic * ND + id) * pks, jw * s);
                 }
             if (a.hybrid != 0.0) {
                 for (int ia = 0; ia < NA; ++ia)
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
             const long long v = wave_sum_i64(__double2ll_rn(jab[k] * 0x1p48));
             if (lane == 0 && v != 0) {
                 if (xr) atomicAdd(xr + (long long)(k / NB) * n + fb + k % NB, (unsigned long long)v);
-                else atomicAdd(Xc + xtri(fa + k / NB, fb + k % NB), (unsigned long long)v);
+                else atomicAdd(a.J + a.jofs[p] + (long long)k * pbs, (unsigned long long)v);
             }
         }
     }
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
             if (v == 0ull) continue;
             const int r = k / n, col = k - r * n;
             const int grow = r < NA ? fa0 + r : fb0 + (r - NA);
-            atomicAdd(Xc + xtri(grow, col), v);
+            atomicAdd(Xc + (long long)grow * n + col, v);
         }
     }
     __syncthreads();
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
     const float X0 = 34.98683665524972f;
     const int n = a.n;
     const int lane = threadIdx.x & 63;
-    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * xtri_size(n);
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * n * n;
     const long long wave0 = __builtin_amdgcn_readfirstlane(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nsurv = 0, nprimq = 0;
@@ -768,7 +768,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                     float s = 0.0f;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ib = 0; ib < NB; ++ib) s += Dp[(long long)(fa + ia) * n + fb + ib] * OUT(ia, ib, ic, id);
-                    fx_add_f(Xc, xtri(fc + ic, fd + id), jw * s);
+                    fx_add_f(a.J, a.jofs[q] + (long long)(ic * ND + id) * pks, jw * s);
                 }
             if (a.hybrid != 0.0) {
                 for (int ia = 0; ia < NA; ++ia)
@@ -807,7 +807,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
             const long long v = wave_sum_i64(__double2ll_rn(double(jab[k]) * 0x1p48));
             if (lane == 0 && v != 0) {
                 if (xr) atomicAdd(xr + (long long)(k / NB) * n + fb + k % NB, (unsigned long long)v);
-                else atomicAdd(Xc + xtri(fa + k / NB, fb + k % NB), (unsigned long long)v);
+                else atomicAdd(a.J + a.jofs[p] + (long long)k * pbs, (unsigned long long)v);
             }
         }
     }
@@ -821,7 +821,7 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
             if (v == 0ull) continue;
             const int r = k / n, col = k - r * n;
             const int grow = r < NA ? fa0 + r : fb0 + (r - NA);
-            atomicAdd(Xc + xtri(grow, col), v);
+            atomicAdd(Xc + (long long)grow * n + col, v);
         }
     }
     __syncthreads();

@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(64, (sizeof(R) == 4 ? 5 : 1)) team_kernel(cons
     const R plim = R(a.prim_limit);
     const int lane = threadIdx.x;
     const int n = a.n;
-    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * xtri_size(n);
+    unsigned long long* const Xc = a.X + (long long)(blockIdx.x % a.xcopies) * n * n;
     const uint4* nodes = TT::nodes();
     const int* lev = TT::levels();
     const unsigned short* tgt = TT::targets();
@@ -286,37 +286,37 @@ __global__ void __launch_bounds__(64, (sizeof(R) == 4 ? 5 : 1)) team_kernel(cons
                     const int ia = e / NB, ib = e % NB;
                     for (int ic = 0; ic < NC; ++ic)
                         for (int id = 0; id < ND; ++id) s += Dcd[ic * ND + id] * OUT(ia, ib, ic, id);
-                    idx = xtri(fa + ia, fb + ib);
+                    idx = (long long)(fa + ia) * n + fb + ib;
                     wt = jw;
                 } else if (e < E2) {
                     const int ic = (e - E1) / ND, id = (e - E1) % ND;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ib = 0; ib < NB; ++ib) s += Dab[ia * NB + ib] * OUT(ia, ib, ic, id);
-                    idx = xtri(fc + ic, fd + id);
+                    idx = (long long)(fc + ic) * n + fd + id;
                     wt = jw;
                 } else if (e < E3) {
                     const int ia = (e - E2) / NC, ic = (e - E2) % NC;
                     for (int ib = 0; ib < NB; ++ib)
                         for (int id = 0; id < ND; ++id) s += Dbd[ib * ND + id] * OUT(ia, ib, ic, id);
-                    idx = xtri(fa + ia, fc + ic);
+                    idx = (long long)(fa + ia) * n + fc + ic;
                     wt = kw;
                 } else if (e < E4) {
                     const int ia = (e - E3) / ND, id = (e - E3) % ND;
                     for (int ib = 0; ib < NB; ++ib)
                         for (int ic = 0; ic < NC; ++ic) s += Dbc[ib * NC + ic] * OUT(ia, ib, ic, id);
-                    idx = xtri(fa + ia, fd + id);
+                    idx = (long long)(fa + ia) * n + fd + id;
                     wt = kw;
                 } else if (e < E5) {
                     const int ib = (e - E4) / NC, ic = (e - E4) % NC;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int id = 0; id < ND; ++id) s += Dad[ia * ND + id] * OUT(ia, ib, ic, id);
-                    idx = xtri(fb + ib, fc + ic);
+                    idx = (long long)(fb + ib) * n + fc + ic;
                     wt = kw;
                 } else {
                     const int ib = (e - E5) / ND, id = (e - E5) % ND;
                     for (int ia = 0; ia < NA; ++ia)
                         for (int ic = 0; ic < NC; ++ic) s += Dac[ia * NC + ic] * OUT(ia, ib, ic, id);
-                    idx = xtri(fb + ib, fd + id);
+                    idx = (long long)(fb + ib) * n + fd + id;
                     wt = kw;
                 }
                 fx_add(Xc, idx, double(wt * s));
