@@ -141,13 +141,16 @@ int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, vo
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory during the call (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): qk_fock_device with its per-build scratch
- * (one int64 accumulator, counters, scaled density, shell-block density maxima)
- * in a fixed device array of 48 MiB that each GPU holds until the process ends,
+ * (one int64 accumulator, the pair-indexed Coulomb buffer, counters, scaled
+ * density, shell-block density maxima) in a fixed device array of 64 MiB that
+ * each GPU holds until the process ends,
  * and no hipMalloc, hipFree or allocating library call.  dG is bitwise the
  * operator qk_fock_device writes.  Requires the work lists for the same options
  * on the device (qk_prepare, or an earlier build): -6 otherwise; -7 when the
- * system needs more scratch than the array holds (with 0.4 shells per function,
- * as 6-31G** water: 1946 functions in FP64, 1706 in mixed precision).  Calls that overlap on one GPU (other streams, other
+ * system needs more scratch than the array holds: about 20 n^2 + 8 nshell^2
+ * bytes in FP64 and 24 n^2 + 8 nshell^2 in mixed precision when every shell pair
+ * is significant (6-31G** water: 1770 and 1620 functions), less when pairs are
+ * screened out.  Calls that overlap on one GPU (other streams, other
  * contexts) share the array and race.  The per-class statistics and the range
  * flag of this build are kept in the array, not reported. */
 int qk_fock_device_noalloc(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream);
