@@ -20,12 +20,15 @@ Legs:
     C20H42 (510 bf), seconds per gradient: FP64, and mixed precision with the
     FP32 threshold swept, the fastest whose max |dgrad| against FP64 is
     <= 1e-6 Eh/bohr reported (with Fock build + gradient, FP64 against mixed);
+  * (H2O)64 B3LYP/6-31G** (1,600 bf): the two-electron build at exchange
+    weight 0.2 (FP64 and mixed), the XC step on the 75 x 302 grid (FP64 MFMA
+    contractions) and the four-centre gradient (b3lyp_water64);
   * the reference CPU path (QUICK's Fortran built from source into
     oracle/_ref) timed on a bounded sample of the same build (cpu_baseline).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--thresholds 1e-3,1e-4,...|none] [--grad-thresholds 1e-1,...]
-                       [--no-gradient] [--copy-kernel-mib N]
+                       [--no-gradient] [--no-water64] [--copy-kernel-mib N]
 """
 from __future__ import annotations
 
@@ -90,6 +93,81 @@ def cpu_baseline(symbols, xyz_ang, D, stride=16, timeout=300):
             "seconds_per_build_est": est}
 
 
+def water64_leg(args, torch, dev, stream, sptr):
+    """(H2O)64 B3LYP/6-31G** (1,600 bf, seed 5): the two-electron build at exchange
+    weight 0.2 (FP64 and mixed at 1e-3), the XC step on the 75 x 302 product grid
+    (FP64 MFMA contractions; a non-negative diagonal density with a known
+    integral), and the four-centre gradient (FP64 and mixed at 1e-2)."""
+    from quick_amd.basis import build_basis
+    from quick_amd.fock import FockEngine, make_opts, QK_FP64, QK_MIXED
+    from quick_amd.geometry import CONFIGS, synthetic_density
+    from quick_amd.xc import QK_GRID_PRODUCT, XcEngine, make_xc_opts
+    syms, xyz = CONFIGS["water64"]()
+    b = build_basis(syms, xyz)
+    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, DENSITY_SEED)
+    eng = FockEngine(b)
+    t0 = time.perf_counter()
+    eng.compute_schwarz()
+    out = {"atoms": len(syms), "nbasis": int(b.nbasis), "schwarz_s": time.perf_counter() - t0}
+    Dd = torch.from_numpy(np.ascontiguousarray(D.ravel(order="F"))).to(dev)
+    Gd = torch.empty_like(Dd)
+    reps = max(1, min(args.steps, 3))
+
+    def evtime(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    Gs = {}
+    for lab, opts in (("fp64", make_opts(hybrid=0.2, precision=QK_FP64)),
+                      ("mixed", make_opts(hybrid=0.2, precision=QK_MIXED, fp32_threshold=1e-3))):
+        eng.prepare(opts)
+        eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
+        torch.cuda.synchronize()
+        msb = evtime(lambda: eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr), reps)
+        Gs[lab] = Gd.cpu().numpy().reshape(b.nbasis, b.nbasis, order="F")
+        st = eng.stats()
+        out[f"fock_{lab}_ms"] = msb
+        out[f"fock_{lab}_quartets"] = int(st.quartets)
+        log(f"(H2O)64 hybrid 0.2 {lab}: {msb:.2f} ms/build, {st.quartets} quartets")
+    out["fock_mixed_abs_dE_vs_fp64"] = abs(0.5 * float(np.sum(D * (Gs["mixed"] - Gs["fp64"]))))
+    out["fock_mixed_max_abs_dF_vs_fp64"] = float(np.abs(Gs["mixed"] - Gs["fp64"]).max())
+    # XC on the 75 x 302 grid
+    rng = np.random.default_rng(3)
+    dg = rng.uniform(0.2, 1.0, b.nbasis)
+    dg *= b.nelec / dg.sum()
+    Dx = torch.from_numpy(np.ascontiguousarray(np.diag(dg).ravel(order="F"))).to(dev)
+    Vx = torch.empty_like(Dx)
+    xe = XcEngine(b)
+    t0 = time.perf_counter()
+    npts = xe.build_grid(make_xc_opts(grid=QK_GRID_PRODUCT, nrad=75, nang=302))
+    grid_s = time.perf_counter() - t0
+    nb, mf, flops = xe.batch_info()
+    E, N = xe.xc_device(Dx.data_ptr(), Vx.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    msx = evtime(lambda: xe.xc_device(Dx.data_ptr(), Vx.data_ptr(), sptr, want_scalars=False), reps)
+    out["xc"] = {"grid": "75 radial x 302 Lebedev per atom, SSF weights", "points": int(npts),
+                 "grid_build_s": grid_s, "batches": nb, "mean_functions_per_batch": mf, "ms": msx,
+                 "mfma_gflop": flops / 1e9, "mfma_tflops_over_whole_step": flops / (msx * 1e-3) / 1e12,
+                 "electrons": N, "electrons_exact": int(b.nelec), "E_xc": E}
+    log(f"(H2O)64 XC: {npts} points, grid {grid_s:.2f} s, {msx:.2f} ms per evaluation, "
+        f"{flops / 1e9:.0f} GFLOP MFMA ({flops / (msx * 1e-3) / 1e12:.1f} TFLOP/s over the step), N {N:.6f}")
+    # four-centre gradient
+    for lab, opts in (("fp64", make_opts(hybrid=0.2, precision=QK_FP64)),
+                      ("mixed", make_opts(hybrid=0.2, precision=QK_MIXED, fp32_threshold=1e-2))):
+        g = torch.zeros(3 * eng.natom, dtype=torch.float64, device=dev)
+        eng.grad_device(Dd.data_ptr(), g.data_ptr(), opts, sptr)
+        torch.cuda.synchronize()
+        out[f"grad_{lab}_ms"] = evtime(lambda: eng.grad_device(Dd.data_ptr(), g.data_ptr(), opts, sptr), 1)
+        log(f"(H2O)64 gradient {lab}: {out[f'grad_{lab}_ms']:.1f} ms")
+    xe.close() if hasattr(xe, "close") else None
+    return out
+
+
 def kernel_of(tag):
     """name of the Fock kernel that evaluates quartet class `tag` (as rocprofv3 lists it, short form)"""
     return "team_kernel" if tag in ("dpdp", "ddpp", "ddds", "dddp", "dddd") else "fock_kernel"
@@ -107,6 +185,9 @@ def main():
     ap.add_argument("--grad-thresholds", default="1e-1,1e-2,1e-3",
                     help="FP32 thresholds of the mixed-precision gradient sweep; the fastest whose max "
                          "|dgrad| against the FP64 gradient is <= 1e-6 Eh/bohr is reported")
+    ap.add_argument("--no-water64", action="store_true",
+                    help="skip the (H2O)64 B3LYP leg (1,600 bf: ERI build at exchange weight 0.2, XC on the "
+                         "75 x 302 grid, four-centre gradient)")
     ap.add_argument("--copy-kernel-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (5 times), to "
                          "calibrate the FETCH_SIZE/WRITE_SIZE counters, and exit")
@@ -300,6 +381,8 @@ def main():
                         "mixed": gpick, "mixed_sweep": mixed,
                         "mixed_vs_fp64_speedup": f64["seconds"] / gpick["seconds"] if gpick else None}
 
+    b3lyp = None if args.no_water64 else water64_leg(args, torch, dev, stream, sptr)
+
     # HBM bytes per launch of the roofline kernel: FETCH_SIZE + WRITE_SIZE from
     # rocprofv3 --pmc passes over this workload, scaled by a plain copy's
     # readings (profiles/pmc_traffic_water32.json); null when the file has no
@@ -338,6 +421,7 @@ def main():
                    "E2_half_trDG_fp64": E64,
                    "gradient_s": grads["water32"]["seconds"] if grads else None},
         "gradient": grads,
+        "b3lyp_water64": b3lyp,
         "fock_plus_gradient_water32": None if not grads.get("water32", {}).get("mixed") else {
             "fp64_ms": ms64 + grads["water32"]["seconds"] * 1e3,
             "mixed_ms": ms + grads["water32"]["mixed"]["seconds"] * 1e3,

@@ -910,19 +910,31 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     // Kernels that spill get a grid small enough that their private-segment
     // backing stays under kScratchBudget (scratch_grid_cap); the grid-stride
     // loop over tiles covers the rest.
-    constexpr size_t kRowsLdsMax = 64 * 1024;   // LDS for one bra's exchange rows
+    // LDS for one bra's exchange rows.  gfx950 gives a block up to 160 KB of
+    // dynamic LDS (152 KB leave room for the bra slab and the kernels' static
+    // LDS, <= 6 KB), but rows beyond 64 KB hold a CU to one or two blocks and
+    // every run zeroes and flushes them: measured on (H2O)64 (1,600 functions)
+    // against direct atomics into X, run mode with large rows pays for the FP64
+    // classes up to a d-p bra (ppps 16.5 -> 10.7 ms, dpps 17.4 -> 9.5 ms, FP64
+    // build 235 -> 202 ms) and for the FP32 classes with a p or p-p ket (ppps
+    // 10.3 -> 7.6 ms), and costs the FP32 classes with an s-s ket (dsss 6.6 ->
+    // 9.1 ms) and every d-d bra (on (H2O)32, ddps 1.04 -> 1.77 ms)
+    constexpr size_t kRowsLdsMax = 152 * 1024, kRowsLdsSmall = 64 * 1024;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
         if (CH.tiles.empty() && CH.tiles32.empty()) continue;
         // one bra pair per block with its exchange rows in LDS, when they fit
         const int nrow = ncart_of(kClassL[k][0]) + ncart_of(kClassL[k][1]);
         const size_t rows_lds = sizeof(unsigned long long) * (size_t)nrow * n;
-        const bool use_runs = !fock_class_is_team(k) && rows_lds <= kRowsLdsMax;
+        const bool big_ok = nrow <= 9 && rows_lds <= kRowsLdsMax;
+        const int nket = ncart_of(kClassL[k][2]) * ncart_of(kClassL[k][3]);
+        const bool use_runs = !fock_class_is_team(k) && (rows_lds <= kRowsLdsSmall || big_ok);
+        const bool use_runs32 = !fock_class_is_team(k) && (rows_lds <= kRowsLdsSmall || (big_ok && nket >= 3));
         // dynamic LDS of the thread-per-quartet kernels: the exchange rows and
         // one bra slab (run mode), or four bra slabs (7 x 64 values each)
         const size_t slab64 = 7 * 64 * sizeof(double), slab32 = 7 * 64 * sizeof(float);
         const size_t lds64 = use_runs ? rows_lds + slab64 : 4 * slab64;
-        const size_t lds32 = use_runs ? rows_lds + slab32 : 4 * slab32;
+        const size_t lds32 = use_runs32 ? rows_lds + slab32 : 4 * slab32;
         if (!CH.tiles.empty()) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
             a.counters = fb->counters + 2 * k;
@@ -962,7 +974,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             size_t lds = lds32;
             // one 64-lane block per tile; up to 20 resident per CU (five wavefronts per SIMD)
             if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 20);
-            if (use_runs && CH.compact32 && fb->compact) {
+            if (use_runs32 && CH.compact32 && fb->compact) {
                 launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
                 w.runs = CH.d_segruns32.p;
                 w.nruns = (long long)CH.segruns32.size();
@@ -971,7 +983,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 w.counts = CH.d_counts32.p;
                 grid = (int)std::min<long long>(w.nruns, cap);
                 lds = lds32;
-            } else if (use_runs) {
+            } else if (use_runs32) {
                 w.runs = CH.d_runs32.p;
                 w.nruns = (long long)CH.runs32.size();
                 grid = (int)std::min<long long>(w.nruns, cap);

@@ -7,7 +7,7 @@ import pytest
 
 from oracle.oracle import Oracle
 from quick_amd.basis import build_basis
-from quick_amd.fock import FockEngine, make_opts, QK_BOYS_ORDER_QSHELL
+from quick_amd.fock import FockEngine, make_opts, QK_BOYS_ORDER_QSHELL, QK_MIXED
 from quick_amd.geometry import CONFIGS, small_system
 
 pytestmark = pytest.mark.gpu
@@ -158,3 +158,40 @@ def test_water32_full_size_matches_reference_samples(golden):
     assert dg <= TOL_F and dd <= TOL_F
     assert dn <= 1e-9
     assert dE <= TOL_E
+
+
+def test_water64_full_size_matches_reference_samples():
+    """(H2O)64, 1,600 functions (the B3LYP configuration's ERI half): the
+    device's Schwarz bounds and G(D) against the reference CPU path's own full
+    build (tests/golden/water64.npz: 40k seeded samples of Ycutoff and of G at a
+    seeded density, the diagonals, row norms and E2), FP64, with every class's
+    exchange rows in LDS; then at the configuration's exchange weight 0.2 the
+    mixed-precision build (threshold 1e-3) against the FP64 one."""
+    from quick_amd.geometry import synthetic_density
+    _banner("water64 vs reference samples")
+    z = np.load("tests/golden/water64.npz")
+    syms, xyz = CONFIGS["water64"]()
+    b = build_basis(syms, xyz)
+    b.xyz = np.asarray(z["xyz"], np.float64).copy()
+    assert b.nbasis == 1600 and int(z["nbasis"]) == 1600
+    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, int(z["density_seed"]))
+    eng = FockEngine(b)
+    Y, _ = eng.compute_schwarz()
+    assert np.abs(Y[z["Y_sample_i"], z["Y_sample_j"]] - z["Y_sample"]).max() <= 1e-12
+    assert np.abs(np.diag(Y) - z["Y_diag"]).max() <= 1e-12
+    G = eng.fock(D)
+    dg = np.abs(G[z["G_sample_i"], z["G_sample_j"]] - z["G_sample"]).max()
+    dd = np.abs(np.diag(G) - z["G_diag"]).max()
+    dn = np.abs(np.linalg.norm(G, axis=1) - z["G_rownorm"]).max()
+    dE = abs(0.5 * np.sum(D * G) - float(z["E2"]))
+    print(f"water64: max|dG| samples {dg:.2e} diag {dd:.2e} rownorm {dn:.2e} |dE2| {dE:.2e}", flush=True)
+    assert dg <= TOL_F and dd <= TOL_F
+    assert dn <= 1e-9
+    assert dE <= TOL_E
+    assert np.array_equal(G, G.T)
+    G64 = eng.fock(D, make_opts(hybrid=0.2))
+    Gm = eng.fock(D, make_opts(hybrid=0.2, precision=QK_MIXED, fp32_threshold=1e-3))
+    dEm = abs(0.5 * np.sum(D * (Gm - G64)))
+    dFm = np.abs(Gm - G64).max()
+    print(f"water64 hybrid 0.2: mixed 1e-3 vs FP64 |dE2| {dEm:.2e} max|dF| {dFm:.2e}", flush=True)
+    assert dEm <= 1e-6 and dFm <= 1e-6
