@@ -6,9 +6,11 @@ cluster (4x4x2 grid, 3.1 A spacing, +-0.3 A jitter, seed 2), Schwarz cutoff
 tau = 1e-11 (primitive cutoff 1e-20).  One "step" is one two-electron operator
 build G(D) for a fixed density: the density (n x n) and the operator are
 resident in HBM; the step runs the density prep, every quartet class kernel and
-the fixed-point finalisation.  The density is a seeded synthetic one (entries
-decaying as exp(-0.25 R^2) between function centres), so that the density
-screen behaves as for a localised SCF density.
+the fixed-point finalisation.  The density is the converged RHF density of the
+reference's own SCF on this geometry (QUICK's Fortran built from source, run to
+DENSERMS 1e-7; tests/golden/water32_scf.npz); --density synthetic takes a seeded
+synthetic one instead (entries decaying as exp(-0.25 R^2) between function
+centres).
 
 Legs:
   * all-FP64 builds (the reference-parity path);
@@ -185,6 +187,9 @@ def main():
     ap.add_argument("--grad-thresholds", default="1e-1,1e-2,1e-3",
                     help="FP32 thresholds of the mixed-precision gradient sweep; the fastest whose max "
                          "|dgrad| against the FP64 gradient is <= 1e-6 Eh/bohr is reported")
+    ap.add_argument("--density", choices=("converged", "synthetic"), default="converged",
+                    help="(H2O)32 density: the reference's converged RHF density (tests/golden/water32_scf.npz) "
+                         "or the seeded synthetic one")
     ap.add_argument("--no-water64", action="store_true",
                     help="skip the (H2O)64 B3LYP leg (1,600 bf: ERI build at exchange weight 0.2, XC on the "
                          "75 x 302 grid, four-centre gradient)")
@@ -213,7 +218,14 @@ def main():
 
     syms, xyz = CONFIGS["water32"]()
     b = build_basis(syms, xyz)
-    D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, DENSITY_SEED)
+    if args.density == "converged":
+        # the reference's own SCF on this geometry (QUICK's Fortran, DENSERMS 1e-7):
+        # D = 2 C_occ C_occ^T, on the coordinates exactly as the reference read them
+        zc = np.load(ROOT / "tests" / "golden" / "water32_scf.npz")
+        b.xyz = np.asarray(zc["xyz"], np.float64).copy()
+        D = 2.0 * zc["C_occ"] @ zc["C_occ"].T
+    else:
+        D = synthetic_density(b.xyz, b.katom, b.ksumtype, b.nbasis, DENSITY_SEED)
     eng = FockEngine(b)
     t0 = time.time()
     eng.compute_schwarz()
@@ -410,13 +422,17 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f32+f64" if precision == "mixed" else "f64",
-        "data": "synthetic: seeded (H2O)32 geometry, seeded exp(-0.25 R^2) density",
+        "data": ("seeded (H2O)32 geometry with the reference's converged RHF density "
+                 "(tests/golden/water32_scf.npz); n-icosane and (H2O)64: seeded synthetic densities"
+                 if args.density == "converged" else
+                 "synthetic: seeded (H2O)32 geometry, seeded exp(-0.25 R^2) density"),
         "config": {"workload": "(H2O)32 HF/6-31G** two-electron Fock build G(D), 800 bf, tau 1e-11, "
                                + ("mixed FP32/FP64 ERIs (FP32 below Schwarz bound %g), FP64 accumulation" % thr
                                   if precision == "mixed" else "all-FP64 ERIs"),
                    "atoms": len(syms), "nbasis": int(b.nbasis),
                    "precision": precision, "fp32_threshold": thr, "geometry_seed": 2,
-                   "density_seed": DENSITY_SEED, "quartets": int(st.quartets),
+                   "density": args.density, "density_seed": DENSITY_SEED if args.density == "synthetic" else None,
+                   "quartets": int(st.quartets),
                    "quartets_fp32": int(st.quartets_fp32), "prim_quartets": int(st.prim_quartets),
                    "E2_half_trDG_fp64": E64,
                    "gradient_s": grads["water32"]["seconds"] if grads else None},
