@@ -905,6 +905,15 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int x, int y) { return c->classes[x].total > c->classes[y].total; });
     if (mixed && !c->d_boys_tab.p) { set_err("mixed precision: Boys table missing"); return -1; }
+    // In a mixed build the FP64 lists hold the few strongest pairs (long ket
+    // ranges on few bras): those kernels run on a second stream beside the FP32
+    // lists, forked and joined with events
+    hipStream_t s64 = s;
+    if (mixed) {
+        s64 = c->streams[0];
+        HIPCHK(hipEventRecord(c->ev_start, s));
+        HIPCHK(hipStreamWaitEvent(s64, c->ev_start, 0));
+    }
     // All classes run one after another on the caller's stream (one hardware
     // queue: kernels that spill to scratch never run concurrently with others).
     // Kernels that spill get a grid small enough that their private-segment
@@ -936,32 +945,32 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         const size_t lds64 = use_runs ? rows_lds + slab64 : 4 * slab64;
         const size_t lds32 = use_runs32 ? rows_lds + slab32 : 4 * slab32;
         if (!CH.tiles.empty()) {
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s64));
             a.counters = fb->counters + 2 * k;
             if (fock_class_is_team(k)) {
                 ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
                 // one 64-lane block per tile; up to 16 resident per CU
                 const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
-                launch_team_class(k, a, w, grid, s);
+                launch_team_class(k, a, w, grid, s64);
             } else {
                 const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch(k), 4));
                 if (use_runs && CH.compact && fb->compact) {
                     // density screen first: every lane of the class kernel then has a quartet
-                    launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s);
+                    launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s64);
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
                                 (long long)CH.segruns.size(), CH.d_segs.p, CH.d_list.p, CH.d_counts.p};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), lds64, s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), lds64, s64);
                 } else if (use_runs) {
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_runs.p, (long long)CH.runs.size()};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), lds64, s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), lds64, s64);
                 } else {
                     ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
                     const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
-                    launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), lds64, s);
+                    launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), lds64, s64);
                 }
             }
             HIPCHK(hipGetLastError());
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s64));
             c->cls_timed[k] = c->timing;
         }
         if (mixed && !CH.tiles32.empty()) {
@@ -995,6 +1004,10 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
             c->cls_timed[slot] = c->timing;
         }
+    }
+    if (mixed) {
+        HIPCHK(hipEventRecord(c->ev_cls[0], s64));
+        HIPCHK(hipStreamWaitEvent(s, c->ev_cls[0], 0));
     }
     launch_jpair_scatter(fb->J, c->dp_jofs.p, c->dp_pstride.p, c->dp_qa.p, c->dp_qb.p, a.q, c->nclass_pairs, fb->X,
                          n, s);
@@ -1473,15 +1486,27 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     ca.grad = 1;
     ca.steps = nullptr;
     ca.slot0 = nullptr;
+    // A mixed gradient's FP64 list holds the few strongest pairs: its kernels
+    // have little parallelism and long per-quartet chains, so they run on a
+    // second stream beside the FP32 list (forked and joined with events).  Only
+    // the kernels of one stream at a time may spill, each within its grid's
+    // scratch bound, so at most two bounded backings are live.
+    hipStream_t s64 = s;
+    if (mixed) {
+        s64 = c->streams[0];
+        HIPCHK(hipEventRecord(c->ev_start, s));
+        HIPCHK(hipStreamWaitEvent(s64, c->ev_start, 0));
+    }
     for (int pass = 0; pass < (mixed ? 2 : 1); ++pass) {
         const bool fp32 = pass == 1;
+        const hipStream_t ls = fp32 ? s : s64;
         for (int k = 0; k < kNumQuartetClasses; ++k) {
             const std::vector<Tile>& T = fp32 ? c->gtiles32[k] : c->gtiles[k];
             if (T.empty()) continue;
             ClassWork w{(fp32 ? c->d_gtiles32[k] : c->d_gtiles[k]).p, (long long)T.size()};
             a.counters = c->d_gcounters.p + 2 * (fp32 ? kNumQuartetClasses + k : k);
             const int slot = (fp32 ? 3 : 2) * kNumQuartetClasses + k;
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], ls));
             if (grad_class_is_light(k) && grad_compacts(k, fp32, mixed)) {
                 // density compaction, then the kernel over the queued steps
                 const int f = fp32 ? 1 : 0;
@@ -1489,17 +1514,17 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 cq.steps = c->d_gsteps[f][k].p;
                 cq.slot0 = c->d_gslot0[f][k].p;
                 launch_compact(cq, c->d_gsegs[f][k].p, (long long)c->gsegs[f][k].size(), c->d_glist[f][k].p,
-                               c->d_gcounts[f][k].p, s);
+                               c->d_gcounts[f][k].p, ls);
                 w.list = c->d_glist[f][k].p;
                 w.steps = cq.steps;
                 const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
-                if (launch_grad_light(k, a, w, grid, fp32, s)) {
+                if (launch_grad_light(k, a, w, grid, fp32, ls)) {
                     set_err("gradient kernel launch failed for class %d", k);
                     return -1;
                 }
             } else if (grad_class_is_light(k)) {
                 const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
-                if (launch_grad_light(k, a, w, grid, fp32, s)) {
+                if (launch_grad_light(k, a, w, grid, fp32, ls)) {
                     set_err("gradient kernel launch failed for class %d", k);
                     return -1;
                 }
@@ -1511,14 +1536,18 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 // (H2O)32): it runs the FP32 form built for four
                 const bool dppp = kClassL[k][0] == 2 && kClassL[k][1] == 1 && kClassL[k][2] == 1 && kClassL[k][3] == 1;
                 const int grid = (int)std::min<long long>((long long)T.size(), 256LL * (fp32 && !dppp ? 20 : 16));
-                if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, dppp, s)) {
+                if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, dppp, ls)) {
                     set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
                     return -1;
                 }
             }
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], ls));
             c->cls_timed[slot] = c->timing;
         }
+    }
+    if (mixed) {
+        HIPCHK(hipEventRecord(c->ev_cls[0], s64));
+        HIPCHK(hipStreamWaitEvent(s, c->ev_cls[0], 0));
     }
     launch_grad_finalize(c->d_gacc.p, dgrad, 3 * c->natom, c->d_gcounters.p + kGradRangeFlag, s);
     HIPCHK(hipGetLastError());
