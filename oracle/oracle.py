@@ -16,7 +16,7 @@ LIB = HERE / "liboracle.so"
 
 
 def build(force: bool = False) -> Path:
-    srcs = [HERE / "qk_oracle.c", HERE / "qk_xc_oracle.c"]
+    srcs = [HERE / "qk_oracle.c", HERE / "qk_xc_oracle.c", HERE / "qk_oneint_oracle.c"]
     if force or not LIB.exists() or any(LIB.stat().st_mtime < s.stat().st_mtime for s in srcs):
         subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", str(LIB),
                         *map(str, srcs), "-lm"], check=True)
@@ -57,6 +57,8 @@ def lib():
         L.qkx_sg1_grid.argtypes = [C.c_int, dp, ip, ip, C.POINTER(dptr), C.c_double, dp, C.c_long]
         L.qkx_xc.argtypes = [C.c_int, dp, C.c_int, ip, ip, ip, dp, dp, dp, dp, C.c_long, dp, C.c_double, dp,
                              dptr, dptr, dptr]
+        L.qko1_hcore.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp]
+        L.qko1_grad.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -198,3 +200,39 @@ def xc(b, D, pts, sig2=None, dmcut: float = 1e-10):
                  sig2, _f(D, np.float64), len(p5), np.ascontiguousarray(p5.ravel()), dmcut, V, C.byref(E),
                  C.byref(a), C.byref(bb))
     return V.reshape(n, n, order="F"), E.value, a.value, bb.value
+
+
+# ---------------------------------------------------------------- one-electron
+
+
+def _oneint_arrays(b):
+    """per-function arrays of a Basis: atoms, (l, m, n), contraction lengths, the
+    Qfinal of each function's shell, exponents and coefficients (10 x n)"""
+    n = int(b.nbasis)
+    itype = np.ascontiguousarray(np.asarray(b.KLMN, np.int32).T.ravel())
+    qf = np.zeros(n, np.int32)
+    ks = np.asarray(b.ksumtype)
+    for s in range(len(b.Qfinal)):
+        qf[ks[s] - 1:ks[s + 1] - 1] = b.Qfinal[s]
+    Z = np.asarray(b.Z, np.float64)
+    xyz = _f(np.asarray(b.xyz, np.float64), np.float64)
+    maxc = int(np.asarray(b.dcoeff).shape[0])
+    return (len(Z), xyz, np.ascontiguousarray(Z), n, maxc, np.ascontiguousarray(b.ncenter, np.int32), itype,
+            np.ascontiguousarray(b.ncontract, np.int32), qf, _f(b.aexp, np.float64), _f(b.dcoeff, np.float64))
+
+
+def hcore(b):
+    """S and H = T + V (qko1_hcore)"""
+    args = _oneint_arrays(b)
+    n = args[3]
+    S, H = np.zeros(n * n), np.zeros(n * n)
+    lib().qko1_hcore(*args, S, H)
+    return S.reshape(n, n, order="F"), H.reshape(n, n, order="F")
+
+
+def oneint_grad(b, D, W):
+    """nuclear repulsion + one-electron part of dE/dR (3 x natom), qko1_grad"""
+    args = _oneint_arrays(b)
+    g = np.zeros(3 * args[0])
+    lib().qko1_grad(*args, _f(D, np.float64), _f(W, np.float64), g)
+    return g.reshape(3, args[0], order="F")
