@@ -166,13 +166,31 @@ int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
 int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 
+/* One-electron integrals and their gradient terms (get1e, quick_one_electron_
+ * integral.f90:1633-1925; the one-electron part of hfgrad, gradient.f90:40-250).
+ * qk_set_charges: the nuclear charges (natom, host), kept with the system.
+ * qk_oneint_device: dS <- S, dH <- H_core = T + V (n x n device arrays; either
+ *   may be NULL); asynchronous on `stream`.
+ * qk_nuclear_repulsion: E <- sum_{A<B} Z_A Z_B / R_AB (host).
+ * qk_oneint_grad_device: dgrad (3 natom, device) += nuclear repulsion gradient
+ *   + sum D dH/dR - sum W dS/dR for the density dD and the energy-weighted
+ *   density dW = 2 sum_occ e_i c_i c_i^T (n x n device arrays); asynchronous.
+ * qk_oneint_host / qk_oneint_grad_host: the same on host arrays, synchronous. */
+int qk_set_charges(qk_ctx* c, const double* charges);
+int qk_oneint_device(qk_ctx* c, double* dS, double* dH, void* stream);
+int qk_nuclear_repulsion(qk_ctx* c, double* E);
+int qk_oneint_grad_device(qk_ctx* c, const double* dD, const double* dW, double* dgrad, void* stream);
+int qk_oneint_host(qk_ctx* c, double* S, double* H);
+int qk_oneint_grad_host(qk_ctx* c, const double* D, const double* W, double* grad);
+
 /* The forward/backward pair under the names the test harness binds (the
  * torch.autograd.Function there wraps them); host arrays, synchronous.
  *   qk_fock_build: F <- F + G(D), as qk_fock_host (gpu_get2e_, gpu.cu:1806).
  *   qk_fock_grad:  grad <- grad + dE2/dR, E2 = 1/2 sum D G(D), as qk_grad_host
- *                  (gpu_grad_, gpu.cu:1481).  W, the energy-weighted density,
- *                  belongs to the one-electron overlap term (gradient.f90:85-237),
- *                  which this library does not compute: it must be NULL. */
+ *                  (gpu_grad_, gpu.cu:1481), when W is NULL; with W, the
+ *                  energy-weighted density, the complete dE/dR of hfgrad
+ *                  (gradient.f90:40-320): nuclear repulsion, the one-electron
+ *                  terms and the four-centre term (needs qk_set_charges). */
 int qk_fock_build(qk_ctx* c, const double* D, double* F, const qk_opts* o);
 int qk_fock_grad(qk_ctx* c, const double* D, const double* W, double* grad, const qk_opts* o);
 

@@ -42,6 +42,9 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
 void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
+void launch_oneint(const OneFuncs& f, const OneSys& sys, double* S, double* H, hipStream_t s);
+void launch_oneint_grad(const OneFuncs& f, const OneSys& sys, const double* D, const double* W,
+                        unsigned long long* G, hipStream_t s);
 void launch_jpair_scatter(const unsigned long long* J, const int* jofs, const int* pstride, const int* qa,
                           const int* qb, const DevQshells& q, int npair, unsigned long long* X, int n,
                           hipStream_t s);
@@ -306,6 +309,13 @@ struct qk_ctx {
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
+    // one-electron integrals: per basis function its atom, (l, m, n), primitive
+    // count, shell Qfinal, exponents and coefficients (10 per function), and
+    // the nuclear charges (qk_set_charges)
+    DBuf<int> d1_atom, d1_lmn, d1_np, d1_qf;
+    DBuf<double> d1_a, d1_c, d_chg;
+    std::vector<double> chg;
+    DBuf<unsigned long long> d1_acc;
     // pair-indexed Coulomb buffer of the prepared pair list (FockArgs::J)
     DBuf<int> dp_jofs;
     DBuf<unsigned long long> d_J;
@@ -546,6 +556,30 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         std::vector<int> qnc(c->q_L.size());
         for (size_t k = 0; k < qnc.size(); ++k) qnc[k] = ncart_of(c->q_L[k]);
         if (c->dq_nc.upload(qnc)) return -1;
+        // per basis function, for the one-electron integrals: the shell's
+        // exponents and the coefficients QUICK's attrashell applies,
+        // gccoeff(k, ksumtype + L) cons(f) (nuclear.f90:196-221)
+        std::vector<int> fat(nbasis), flmn(nbasis), fnp(nbasis), fqf(nbasis);
+        std::vector<double> fa(10 * (size_t)nbasis, 0.0), fc(10 * (size_t)nbasis, 0.0);
+        for (size_t q = 0; q < c->q_L.size(); ++q) {
+            const int sh = c->q_shell[q], L = c->q_L[q];
+            const int col = ksumtype[sh] - 1 + L;
+            for (int i = 0; i < ncart_of(L); ++i) {
+                const int fn = c->q_f0[q] + i;
+                fat[fn] = katom[sh] - 1;
+                flmn[fn] = KLMN[3 * fn] | (KLMN[3 * fn + 1] << 4) | (KLMN[3 * fn + 2] << 8);
+                fnp[fn] = kprim[sh];
+                fqf[fn] = Qfinal[sh];
+                for (int u = 0; u < kprim[sh]; ++u) {
+                    fa[10 * (size_t)fn + u] = gcexpo[u + 10 * (size_t)col];
+                    fc[10 * (size_t)fn + u] = gccoeff[u + 10 * (size_t)col] * cons[fn];
+                }
+            }
+        }
+        if (c->d1_atom.upload(fat) || c->d1_lmn.upload(flmn) || c->d1_np.upload(fnp) || c->d1_qf.upload(fqf) ||
+            c->d1_a.upload(fa) || c->d1_c.upload(fc))
+            return -1;
+        c->chg.clear();
     }
     size_t nn = (size_t)nbasis * nbasis;
     if (c->d_Dp.alloc(nn) || c->d_X.alloc(nn * kXCopies) || c->d_cm.alloc((size_t)nshell * nshell)) return -1;
@@ -1576,18 +1610,102 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
     return check_range(c->d_gcounters.p + kGradRangeFlag, "qk_grad_host");
 }
 
+// ---------------------------------------------------------------------------
+// one-electron integrals and their gradient terms
+
+extern "C" int qk_set_charges(qk_ctx* c, const double* charges) {
+    if (!c || !c->have_system) { set_err("qk_set_charges: no system"); return -1; }
+    (void)hipSetDevice(c->device);
+    c->chg.assign(charges, charges + c->natom);
+    return c->d_chg.upload(c->chg);
+}
+
+static int oneint_ready(qk_ctx* c, const char* what) {
+    if (!c || !c->have_system) { set_err("%s: no system", what); return -1; }
+    if ((int)c->chg.size() != c->natom) { set_err("%s: nuclear charges not set (qk_set_charges)", what); return -1; }
+    (void)hipSetDevice(c->device);
+    return 0;
+}
+
+static OneFuncs one_funcs(const qk_ctx* c) {
+    return OneFuncs{c->d1_atom.p, c->d1_lmn.p, c->d1_np.p, c->d1_qf.p, c->d1_a.p, c->d1_c.p};
+}
+
+extern "C" int qk_oneint_device(qk_ctx* c, double* dS, double* dH, void* stream) {
+    if (oneint_ready(c, "qk_oneint_device")) return -1;
+    const OneSys sys{c->d_xyz.p, c->d_chg.p, c->natom, c->nbasis};
+    launch_oneint(one_funcs(c), sys, dS, dH, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int qk_nuclear_repulsion(qk_ctx* c, double* E) {
+    if (oneint_ready(c, "qk_nuclear_repulsion")) return -1;
+    double e = 0.0;
+    for (int i = 0; i < c->natom; ++i)
+        for (int j = i + 1; j < c->natom; ++j) {
+            const double dx = c->xyz[3 * i] - c->xyz[3 * j], dy = c->xyz[3 * i + 1] - c->xyz[3 * j + 1],
+                         dz = c->xyz[3 * i + 2] - c->xyz[3 * j + 2];
+            e += c->chg[i] * c->chg[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    *E = e;
+    return 0;
+}
+
+extern "C" int qk_oneint_grad_device(qk_ctx* c, const double* dD, const double* dW, double* dgrad, void* stream) {
+    if (oneint_ready(c, "qk_oneint_grad_device")) return -1;
+    hipStream_t s = (hipStream_t)stream;
+    const int m = 3 * c->natom;
+    if (8LL * m > 150 * 1024) {   // the kernel's per-block LDS copy of the gradient
+        set_err("qk_oneint_grad_device: %d atoms; at most 6400 are supported", c->natom);
+        return -2;
+    }
+    if (c->d1_acc.alloc(m + 1)) return -1;
+    launch_fill_u64(c->d1_acc.p, m + 1, s);
+    const OneSys sys{c->d_xyz.p, c->d_chg.p, c->natom, c->nbasis};
+    launch_oneint_grad(one_funcs(c), sys, dD, dW, c->d1_acc.p, s);
+    launch_grad_finalize(c->d1_acc.p, dgrad, m, c->d1_acc.p + m, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int qk_oneint_host(qk_ctx* c, double* S, double* H) {
+    if (oneint_ready(c, "qk_oneint_host")) return -1;
+    const size_t nn = (size_t)c->nbasis * c->nbasis;
+    DBuf<double> dS, dH;
+    if (dS.alloc(nn) || dH.alloc(nn)) return -1;
+    if (qk_oneint_device(c, dS.p, dH.p, nullptr)) return -1;
+    if (S) HIPCHK(hipMemcpy(S, dS.p, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (H) HIPCHK(hipMemcpy(H, dH.p, nn * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int qk_oneint_grad_host(qk_ctx* c, const double* D, const double* W, double* grad) {
+    if (oneint_ready(c, "qk_oneint_grad_host")) return -1;
+    const size_t nn = (size_t)c->nbasis * c->nbasis, m = 3 * (size_t)c->natom;
+    DBuf<double> dD, dW, dg;
+    if (dD.alloc(nn) || dW.alloc(nn) || dg.alloc(m)) return -1;
+    HIPCHK(hipMemcpy(dD.p, D, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dW.p, W, nn * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dg.p, grad, m * sizeof(double), hipMemcpyHostToDevice));
+    if (qk_oneint_grad_device(c, dD.p, dW.p, dg.p, nullptr)) return -1;
+    HIPCHK(hipMemcpy(grad, dg.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    return check_range(c->d1_acc.p + m, "qk_oneint_grad_host");
+}
+
 // the forward/backward pair of the test harness (same operations as the
 // host entry points above)
 extern "C" int qk_fock_build(qk_ctx* c, const double* D, double* F, const qk_opts* o) {
     return qk_fock_host(c, D, F, o);
 }
 
+// W NULL: the four-centre term only; W given: the complete dE/dR of hfgrad
+// (nuclear repulsion, one-electron terms with W, four-centre term), which needs
+// the nuclear charges (qk_set_charges)
 extern "C" int qk_fock_grad(qk_ctx* c, const double* D, const double* W, double* grad, const qk_opts* o) {
-    if (W) {
-        set_err("qk_fock_grad: W (energy-weighted density) must be NULL: the one-electron terms are not computed here");
-        return -4;
-    }
-    return qk_grad_host(c, D, grad, o);
+    if (W && oneint_ready(c, "qk_fock_grad")) return -4;
+    if (qk_grad_host(c, D, grad, o)) return -1;
+    return W ? qk_oneint_grad_host(c, D, W, grad) : 0;
 }
 
 // surviving quartets and primitive quartets of class k in the last gradient

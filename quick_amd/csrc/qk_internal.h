@@ -219,6 +219,23 @@ struct GradArgs {
     const float* boys_tab;   // kBoysTabRows x kBoysTabM
 };
 
+// One-electron integrals (qk_oneint.hip): per basis function its data, and the nuclei.
+constexpr int k1eMaxPrim = 10;   // QUICK's MAXPRIM
+struct OneFuncs {
+    const int* atom;     // 0-based
+    const int* lmn;      // l | m << 4 | n << 8
+    const int* np;       // primitives
+    const int* qf;       // Qfinal of the function's shell (Boys order)
+    const double* a;     // k1eMaxPrim exponents per function
+    const double* c;     // k1eMaxPrim coefficients per function: gccoeff(k, ksumtype + L) cons(f)
+};
+struct OneSys {
+    const double* xyz;   // 3 x natom
+    const double* chg;   // nuclear charges
+    int natom;
+    int n;
+};
+
 struct FockArgs {
     DevQshells q;
     DevPairs pr;

@@ -67,6 +67,12 @@ def lib() -> C.CDLL:
         L.qk_prepare.argtypes = [P, C.POINTER(QkOpts)]
         L.qk_fock_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_fock_device_noalloc.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
+        L.qk_set_charges.argtypes = [P, dp]
+        L.qk_oneint_device.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qk_nuclear_repulsion.argtypes = [P, dp]
+        L.qk_oneint_grad_device.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qk_oneint_host.argtypes = [P, dp, dp]
+        L.qk_oneint_grad_host.argtypes = [P, dp, dp, dp]
         L.qk_fock_host.argtypes = [P, dp, dp, C.POINTER(QkOpts)]
         L.qk_grad_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_grad_host.argtypes = [P, dp, dp, C.POINTER(QkOpts)]
@@ -177,6 +183,32 @@ class FockEngine:
         opts = opts or make_opts()
         return int(lib().qk_fock_device_noalloc(self.h, C.c_void_p(D_ptr), C.c_void_p(G_ptr), C.byref(opts),
                                                 C.c_void_p(stream)))
+
+    def set_charges(self, charges):
+        """nuclear charges (natom) for the one-electron integrals"""
+        self._chg = np.ascontiguousarray(np.asarray(charges, np.float64))
+        _check(lib().qk_set_charges(self.h, _ptr(self._chg, C.c_double)), "qk_set_charges")
+
+    def oneint(self):
+        """S and H_core = T + V (n x n, host)"""
+        n = self.nbasis
+        S, H = np.zeros(n * n), np.zeros(n * n)
+        _check(lib().qk_oneint_host(self.h, _ptr(S, C.c_double), _ptr(H, C.c_double)), "qk_oneint_host")
+        return S.reshape(n, n, order="F"), H.reshape(n, n, order="F")
+
+    def nuclear_repulsion(self) -> float:
+        e = C.c_double()
+        _check(lib().qk_nuclear_repulsion(self.h, C.byref(e)), "qk_nuclear_repulsion")
+        return e.value
+
+    def oneint_grad(self, D, W) -> np.ndarray:
+        """nuclear repulsion + one-electron part of dE/dR (3 x natom) for the
+        density D and energy-weighted density W"""
+        Df, Wf = _f(D, np.float64), _f(W, np.float64)
+        g = np.zeros(3 * self.natom)
+        _check(lib().qk_oneint_grad_host(self.h, _ptr(Df, C.c_double), _ptr(Wf, C.c_double), _ptr(g, C.c_double)),
+               "qk_oneint_grad_host")
+        return g.reshape(3, self.natom, order="F")
 
     def fock(self, D, opts: QkOpts | None = None) -> np.ndarray:
         """Host arrays: returns G(D) (n x n)."""

@@ -71,7 +71,7 @@ def test_autograd_mixed_backward_against_fp64():
     assert np.abs(out["mixed"][1] - out["fp64"][1]).max() <= 1e-6
 
 
-def test_fock_grad_rejects_energy_weighted_density():
+def test_fock_grad_with_energy_weighted_density_needs_charges():
     print("[gpu-test] qk_fock_grad W", flush=True)
     syms, xyz, D = _system("h2o")
     eng = FockEngine(build_basis(syms, xyz))
@@ -81,7 +81,7 @@ def test_fock_grad_rejects_energy_weighted_density():
     g = np.zeros(3 * eng.natom)
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
     rc = lib().qk_fock_grad(eng.h, dp(Df), dp(W), dp(g), C.byref(make_opts()))
-    assert rc != 0 and "must be NULL" in last_error()
+    assert rc != 0 and "charges" in last_error()
     assert not g.any()
     # the forward of the pair is the host build
     F = np.zeros_like(Df)
