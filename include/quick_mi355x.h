@@ -183,6 +183,25 @@ int qk_oneint_grad_device(qk_ctx* c, const double* dD, const double* dW, double*
 int qk_oneint_host(qk_ctx* c, double* S, double* H);
 int qk_oneint_grad_host(qk_ctx* c, const double* D, const double* W, double* grad);
 
+/* The RHF SCF step beside the Fock build (electdiis, scf.f90:56-645): the core-
+ * Hamiltonian guess, then per cycle F = H + G(D) from qk_fock_device with the
+ * options in `fock`, E = 1/2 sum D (H + F) + E_nn, Pulay DIIS on F D S - S D F
+ * (up to max_diis stored cycles), X F X diagonalised in FP64 (X = S^{-1/2}),
+ * D = 2 C_occ C_occ^T, until the RMS change of D falls below dens_rms (QUICK's
+ * DENSERMS).  D, F and the DIIS history stay on the device; needs the system,
+ * its Schwarz bounds and the nuclear charges (qk_set_charges).  Outputs (host,
+ * each may be NULL): the final density (n x n), its energy, the cycles run and
+ * the last RMS change.  Returns -4 when the cycle limit is reached unconverged. */
+typedef struct qk_scf_opts {
+    int max_cycles;        /* default 100 */
+    int max_diis;          /* stored cycles for DIIS, 1..10 (default 10) */
+    double dens_rms;       /* convergence: RMS change of D (default 1e-7) */
+    qk_opts fock;          /* options of the Fock builds */
+} qk_scf_opts;
+void qk_scf_default_opts(qk_scf_opts* o);
+int qk_scf(qk_ctx* c, int nocc, const qk_scf_opts* o, double* D_out, double* E_out, int* cycles_out,
+           double* rms_out);
+
 /* The forward/backward pair under the names the test harness binds (the
  * torch.autograd.Function there wraps them); host arrays, synchronous.
  *   qk_fock_build: F <- F + G(D), as qk_fock_host (gpu_get2e_, gpu.cu:1806).

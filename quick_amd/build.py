@@ -61,12 +61,14 @@ def build(verbose: bool = True, jobs: int = 8) -> Path:
                    check=True, stdout=subprocess.DEVNULL)
     srcs = sorted(list((CSRC / "gen").glob("fock_*.hip")) + list((CSRC / "gen").glob("grad_*.hip")),
                   key=lambda p: -p.stat().st_size)
-    srcs += [CSRC / "qk_xc.hip", CSRC / "qk_oneint.hip", CSRC / "qk_fock.hip", CSRC / "qk_grad.hip",
-             CSRC / "qk_host.cpp"]
+    srcs += [CSRC / "qk_xc.hip", CSRC / "qk_oneint.hip", CSRC / "qk_scf.hip", CSRC / "qk_fock.hip",
+             CSRC / "qk_grad.hip", CSRC / "qk_host.cpp"]
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    # rocBLAS and rocSOLVER serve the SCF step (qk_scf.hip)
+    cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     if verbose:

@@ -1610,6 +1610,13 @@ extern "C" int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_o
     return check_range(c->d_gcounters.p + kGradRangeFlag, "qk_grad_host");
 }
 
+int qk_context_info(qk_ctx* c, int* device, int* nbasis) {
+    if (!c || !c->have_system) { set_err("no system"); return -1; }
+    *device = c->device;
+    *nbasis = c->nbasis;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // one-electron integrals and their gradient terms
 

@@ -31,6 +31,10 @@ class QkOpts(C.Structure):
                 ("fp32_threshold", C.c_double)]
 
 
+class QkScfOpts(C.Structure):
+    _fields_ = [("max_cycles", C.c_int), ("max_diis", C.c_int), ("dens_rms", C.c_double), ("fock", QkOpts)]
+
+
 class QkStats(C.Structure):
     _fields_ = [("quartets", C.c_longlong), ("quartets_fp32", C.c_longlong),
                 ("prim_quartets", C.c_longlong), ("flops_fp64", C.c_double),
@@ -68,6 +72,9 @@ def lib() -> C.CDLL:
         L.qk_fock_device.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_fock_device_noalloc.argtypes = [P, C.c_void_p, C.c_void_p, C.POINTER(QkOpts), C.c_void_p]
         L.qk_set_charges.argtypes = [P, dp]
+        L.qk_scf_default_opts.argtypes = [C.POINTER(QkScfOpts)]
+        L.qk_scf.argtypes = [P, C.c_int, C.POINTER(QkScfOpts), dp, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                             C.POINTER(C.c_double)]
         L.qk_oneint_device.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         L.qk_nuclear_repulsion.argtypes = [P, dp]
         L.qk_oneint_grad_device.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -188,6 +195,19 @@ class FockEngine:
         """nuclear charges (natom) for the one-electron integrals"""
         self._chg = np.ascontiguousarray(np.asarray(charges, np.float64))
         _check(lib().qk_set_charges(self.h, _ptr(self._chg, C.c_double)), "qk_set_charges")
+
+    def scf(self, nocc: int, fock_opts: QkOpts | None = None, dens_rms: float = 1e-7, max_cycles: int = 100):
+        """RHF SCF on the device (qk_scf): returns (D, E, cycles, last RMS change of D)"""
+        o = QkScfOpts()
+        lib().qk_scf_default_opts(C.byref(o))
+        o.dens_rms, o.max_cycles = dens_rms, max_cycles
+        if fock_opts is not None:
+            o.fock = fock_opts
+        D = np.zeros(self.nbasis * self.nbasis)
+        E, cyc, rms = C.c_double(), C.c_int(), C.c_double()
+        _check(lib().qk_scf(self.h, int(nocc), C.byref(o), _ptr(D, C.c_double), C.byref(E), C.byref(cyc),
+                            C.byref(rms)), "qk_scf")
+        return D.reshape(self.nbasis, self.nbasis, order="F"), E.value, cyc.value, rms.value
 
     def oneint(self):
         """S and H_core = T + V (n x n, host)"""
