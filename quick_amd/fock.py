@@ -196,8 +196,10 @@ class FockEngine:
         self._chg = np.ascontiguousarray(np.asarray(charges, np.float64))
         _check(lib().qk_set_charges(self.h, _ptr(self._chg, C.c_double)), "qk_set_charges")
 
-    def scf(self, nocc: int, fock_opts: QkOpts | None = None, dens_rms: float = 1e-7, max_cycles: int = 100):
-        """RHF SCF on the device (qk_scf): returns (D, E, cycles, last RMS change of D)"""
+    def scf(self, nocc: int, fock_opts: QkOpts | None = None, dens_rms: float = 1e-7, max_cycles: int = 100,
+            require_convergence: bool = True):
+        """RHF SCF on the device (qk_scf): returns (D, E, cycles, last RMS change of D);
+        an unconverged run at the cycle limit raises unless require_convergence is False"""
         o = QkScfOpts()
         lib().qk_scf_default_opts(C.byref(o))
         o.dens_rms, o.max_cycles = dens_rms, max_cycles
@@ -205,8 +207,10 @@ class FockEngine:
             o.fock = fock_opts
         D = np.zeros(self.nbasis * self.nbasis)
         E, cyc, rms = C.c_double(), C.c_int(), C.c_double()
-        _check(lib().qk_scf(self.h, int(nocc), C.byref(o), _ptr(D, C.c_double), C.byref(E), C.byref(cyc),
-                            C.byref(rms)), "qk_scf")
+        rc = lib().qk_scf(self.h, int(nocc), C.byref(o), _ptr(D, C.c_double), C.byref(E), C.byref(cyc),
+                          C.byref(rms))
+        if not (rc == -4 and not require_convergence):
+            _check(rc, "qk_scf")
         return D.reshape(self.nbasis, self.nbasis, order="F"), E.value, cyc.value, rms.value
 
     def oneint(self):
