@@ -25,12 +25,16 @@ Legs:
   * (H2O)64 B3LYP/6-31G** (1,600 bf): the two-electron build at exchange
     weight 0.2 (FP64 and mixed), the XC step on the 75 x 302 grid (FP64 MFMA
     contractions) and the four-centre gradient (b3lyp_water64);
+  * the (H2O)32 RHF SCF on the device from the core guess (FP64 and mixed Fock
+    builds) against the reference's SCF energy, and the one-electron integrals
+    and their gradient terms (scf_water32);
   * the reference CPU path (QUICK's Fortran built from source into
     oracle/_ref) timed on a bounded sample of the same build (cpu_baseline).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--thresholds 1e-3,1e-4,...|none] [--grad-thresholds 1e-1,...]
-                       [--no-gradient] [--no-water64] [--copy-kernel-mib N]
+                       [--density converged|synthetic] [--no-gradient] [--no-scf]
+                       [--no-water64] [--copy-kernel-mib N]
 """
 from __future__ import annotations
 
@@ -95,11 +99,14 @@ def cpu_baseline(symbols, xyz_ang, D, stride=16, timeout=300):
             "seconds_per_build_est": est}
 
 
-def scf_leg(b, eng, syms):
+def scf_leg(b, eng, syms, torch, dev, stream, sptr):
     """(H2O)32 RHF SCF on the device (qk_scf) from the core-Hamiltonian guess to
     DENSERMS 1e-7, with FP64 and with mixed-precision (1e-3) Fock builds,
-    against the reference's own SCF energy on this geometry."""
-    from quick_amd.fock import QK_MIXED, make_opts
+    against the reference's own SCF energy on this geometry; and the
+    one-electron integrals (S, H_core) and their gradient terms on the device,
+    timed with HIP events on the launch stream."""
+    import ctypes as C
+    from quick_amd.fock import QK_MIXED, lib, make_opts
     zc = np.load(ROOT / "tests" / "golden" / "water32_scf.npz")
     Zof = {"H": 1.0, "C": 6.0, "N": 7.0, "O": 8.0}
     eng.set_charges([Zof[s] for s in syms])
@@ -114,6 +121,29 @@ def scf_leg(b, eng, syms):
         out[lab] = {"seconds": dt, "cycles": cyc, "E": E, "abs_dE_vs_reference": abs(E - out["reference_E"])}
         log(f"(H2O)32 SCF {lab}: {dt:.2f} s, {cyc} cycles, E {E:.10f}, |dE| vs reference "
             f"{abs(E - out['reference_E']):.2e}")
+    # one-electron legs at the reference's converged density D = 2 C_occ C_occ^T
+    # and energy-weighted density W = 2 C_occ diag(eps_occ) C_occ^T
+    L, Co = lib(), np.asarray(zc["C_occ"], np.float64)
+    eps = np.asarray(zc["eps"], np.float64)[:Co.shape[1]]
+    dev_f = lambda M: torch.from_numpy(np.ascontiguousarray(M.ravel(order="F"))).to(dev)
+    Dd, Wd = dev_f(2.0 * Co @ Co.T), dev_f(2.0 * (Co * eps) @ Co.T)
+    Sd, Hd = torch.empty_like(Dd), torch.empty_like(Dd)
+    gd = torch.zeros(3 * len(syms), dtype=torch.float64, device=dev)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    legs = {"oneint_S_H": lambda: L.qk_oneint_device(eng.h, p(Sd), p(Hd), C.c_void_p(sptr)),
+            "oneint_grad": lambda: L.qk_oneint_grad_device(eng.h, p(Dd), p(Wd), p(gd), C.c_void_p(sptr))}
+    for lab, fn in legs.items():
+        if fn() != 0:
+            raise RuntimeError(f"{lab} failed: {L.qk_last_error().decode()}")
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[lab + "_ms"] = e0.elapsed_time(e1) / 5
+        log(f"(H2O)32 {lab}: {out[lab + '_ms']:.3f} ms")
     return out
 
 
@@ -421,7 +451,7 @@ def main():
                         "mixed": gpick, "mixed_sweep": mixed,
                         "mixed_vs_fp64_speedup": f64["seconds"] / gpick["seconds"] if gpick else None}
 
-    scf = None if args.no_scf else scf_leg(b, eng, syms)
+    scf = None if args.no_scf else scf_leg(b, eng, syms, torch, dev, stream, sptr)
     b3lyp = None if args.no_water64 else water64_leg(args, torch, dev, stream, sptr)
 
     # HBM bytes per launch of the roofline kernel: FETCH_SIZE + WRITE_SIZE from
