@@ -61,7 +61,7 @@ def build(verbose: bool = True, jobs: int = 8) -> Path:
                    check=True, stdout=subprocess.DEVNULL)
     srcs = sorted(list((CSRC / "gen").glob("fock_*.hip")) + list((CSRC / "gen").glob("grad_*.hip")),
                   key=lambda p: -p.stat().st_size)
-    srcs += [CSRC / "qk_xc.hip", CSRC / "qk_oneint.hip", CSRC / "qk_scf.hip", CSRC / "qk_fock.hip",
+    srcs += [CSRC / "qk_xc.hip", CSRC / "qk_oneint.hip", CSRC / "qk_scf.hip", CSRC / "qk_lists.hip", CSRC / "qk_fock.hip",
              CSRC / "qk_grad.hip", CSRC / "qk_host.cpp"]
     with ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -50,6 +51,21 @@ void launch_jpair_scatter(const unsigned long long* J, const int* jofs, const in
                           hipStream_t s);
 void launch_fock_finalize(const unsigned long long* X, int copies, const double* cons, double* G, int n,
                           unsigned long long* flag, hipStream_t s);
+size_t list_scan_bytes(long long n);
+int lists_count(const ListEntry* E, int ne, const KetGroup* KG, const ListClass* LC, const double* Y, double tau,
+                double thr, long long N, int4* items, long long* cnt, long long* off, void* tmp, size_t tmp_bytes,
+                const long long* cls_item, int ncls, long long* sums, hipStream_t s);
+void lists_write(const ListEntry* E, int ne, const KetGroup* KG, const ListClass* LC, long long N, const int4* items,
+                 const long long* off, const long long* cls_item, const ListDst* dst, hipStream_t s);
+int lists_runs_count(const ListEntry* E, int ne, const ListClass* LC, long long NB, const int4* items,
+                     long long* rcnt, long long* roff, void* tmp, size_t tmp_bytes, const long long* cls_bra, int ncls,
+                     long long* sums, hipStream_t s);
+void lists_runs_write(const ListEntry* E, int ne, const ListClass* LC, long long NB, const int4* items,
+                      const long long* off, long long N, const long long* cls_item, const long long* roff,
+                      const long long* cls_bra, const ListDst* dst, hipStream_t s);
+void launch_gather_prims(const int* sel, const int* src0, const int* nprim, const int* prim0, const int* pstride,
+                         int npair, const PrimArrays& src, const PrimArrays& dst, const PrimArraysF& f32,
+                         hipStream_t s);
 void launch_schwarz(int pair_class, const int* pairs, int np, const int* qa, const int* qb, const int* prim0,
                     const int* nprim, const DevQshells& qs, const DevPrims& pp, const double* xyz, const double* cons,
                     double* ymax, double* pmax, hipStream_t s);
@@ -138,9 +154,9 @@ static int pair_class_of(int la, int lb) {
 }
 
 struct ClassHost {
-    std::vector<Tile> tiles;     // FP64 work, the tiles of one bra pair consecutive
-    std::vector<Tile> tiles32;   // FP32 work (mixed precision only)
-    std::vector<int2> runs, runs32;   // [first, end) tile of each bra pair
+    // FP64 tiles (the tiles of one bra pair consecutive) and FP32 tiles (mixed
+    // precision only); runs: [first, end) tile of each bra pair
+    long long ntiles = 0, ntiles32 = 0, nruns = 0, nruns32 = 0;
     long long total = 0;         // quartets passing the Schwarz test
     long long total32 = 0;       // of which in the FP32 lists
     DBuf<Tile> d_tiles, d_tiles32;
@@ -148,8 +164,7 @@ struct ClassHost {
     // the same candidates as segments {bra pair, first ket, end ket, offset}
     // (one bra pair and one ket group each), for the per-build density
     // compaction; segruns: [first, end) segment of each bra pair
-    std::vector<int4> segs, segs32;
-    std::vector<int2> segruns, segruns32;
+    long long nsegs = 0, nsegs32 = 0, nsegruns = 0, nsegruns32 = 0;
     DBuf<int4> d_segs, d_segs32;
     DBuf<int2> d_segruns, d_segruns32;
     DBuf<int> d_list, d_list32, d_counts, d_counts32;
@@ -157,66 +172,6 @@ struct ClassHost {
     // segments every step is already the whole segment)
     bool compact = false, compact32 = false;
 };
-
-// runs of tiles sharing one bra pair (one block's unit of work, the bra's
-// exchange rows in LDS, added to X once per run), of at most cap candidate kets
-static void tile_runs(const std::vector<Tile>& T, std::vector<int2>& runs, int cap) {
-    runs.clear();
-    int cands = 0;
-    for (size_t k = 0; k < T.size(); ++k) {
-        if (runs.empty() || T[runs.back().x].p != T[k].p || cands + T[k].len > cap) {
-            runs.push_back({(int)k, (int)k + 1});
-            cands = 0;
-        } else {
-            runs.back().y = (int)k + 1;
-        }
-        cands += T[k].len;
-    }
-}
-
-// the same over compaction segments (push_segment keeps each within cap)
-static void seg_runs(const std::vector<int4>& S, std::vector<int2>& runs, int cap) {
-    runs.clear();
-    int cands = 0;
-    for (size_t k = 0; k < S.size(); ++k) {
-        const int len = S[k].z - S[k].y;
-        if (runs.empty() || S[runs.back().x].x != S[k].x || cands + len > cap) {
-            runs.push_back({(int)k, (int)k + 1});
-            cands = 0;
-        } else {
-            runs.back().y = (int)k + 1;
-        }
-        cands += len;
-    }
-}
-
-// f(k) for every quartet class on up to 16 host threads: the classes' work
-// lists are independent (f touches only class k's vectors and makes no HIP
-// call); classes are taken in index order, the large s/p classes first
-template <class F>
-static void for_each_class(F&& f) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nth = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)kNumQuartetClasses);
-    std::atomic<int> next{0};
-    auto work = [&] {
-        for (int k = next.fetch_add(1); k < kNumQuartetClasses; k = next.fetch_add(1)) f(k);
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nth; ++t) pool.emplace_back(work);
-    work();
-    for (std::thread& th : pool) th.join();
-}
-
-// the candidate kets [q0, q1) of bra pair p as segments of at most cap kets,
-// their compacted lists from offset off on
-static void push_segment(std::vector<int4>& S, int p, int q0, int q1, int cap, long long& off) {
-    for (int q = q0; q < q1;) {
-        const int e = q1 - q > cap ? q + cap : q1;
-        S.push_back({p, q, e, (int)off});
-        off += e - q;
-        q = e;
-    }
-}
 
 // Boys values F_m(k / kBoysTabPerUnit) for the FP32 kernels, from the series
 // F_m(T) = exp(-T) sum_i (2T)^i / ((2m+1)(2m+3)...(2m+2i+1)) in FP64 (all terms
@@ -280,13 +235,12 @@ struct qk_ctx {
 
     // gradient: per-class tables on the device, work lists, accumulators
     DBuf<int> d_gtab[kNumQuartetClasses];
-    std::vector<Tile> gtiles[kNumQuartetClasses], gtiles32[kNumQuartetClasses];   // FP64 / FP32 work
+    long long ngtiles[kNumQuartetClasses] = {}, ngtiles32[kNumQuartetClasses] = {};   // FP64 / FP32 work
     DBuf<Tile> d_gtiles[kNumQuartetClasses], d_gtiles32[kNumQuartetClasses];
     // light classes: segments {bra pair, first ket, end ket, list offset} of the
-    // per-gradient density compaction and their runs per bra pair (index 0 FP64,
-    // 1 FP32)
-    std::vector<int4> gsegs[2][kNumQuartetClasses];
-    std::vector<int> gslot0[2][kNumQuartetClasses];   // first tile of each segment
+    // per-gradient density compaction (index 0 FP64, 1 FP32) with the first
+    // tile of each (d_gslot0)
+    long long ngsegs[2][kNumQuartetClasses] = {};
     DBuf<int4> d_gsegs[2][kNumQuartetClasses];
     DBuf<Tile> d_gsteps[2][kNumQuartetClasses];
     DBuf<int> d_gslot0[2][kNumQuartetClasses];
@@ -305,7 +259,17 @@ struct qk_ctx {
     DBuf<int> dp_qa, dp_qb, dp_prim0, dp_nprim;
     DBuf<int> dp_sh, dp_at;   // per pair: shells and atoms of its two Qshells, 16 bits each (< 2^15)
     DBuf<double> dp_Y;
-    DBuf<double> dpp_zeta, dpp_oo2z, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut;
+    DBuf<double> dpp_zeta, dpp_oo2z, dpp_Px, dpp_Py, dpp_Pz, dpp_K, dpp_cut, dpp_ea, dpp_eb;
+    // work-list construction (qk_lists.hip): the significant pairs' indices into
+    // the all-pairs arrays, and the scratch of the list passes
+    DBuf<int> d_sel;
+    DBuf<ListEntry> dl_E;
+    DBuf<KetGroup> dl_KG;
+    DBuf<ListClass> dl_LC;
+    DBuf<ListDst> dl_dst;
+    DBuf<int4> dl_items;
+    DBuf<long long> dl_ci, dl_cb, dl_cnt, dl_off, dl_sums, dl_rcnt, dl_roff, dl_rsums;
+    DBuf<unsigned char> dl_tmp;
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
@@ -465,7 +429,8 @@ static int build_all_pairs(qk_ctx* c) {
     c->ap_cut.assign(zeta.size(), 0.0);
     (void)hipSetDevice(c->device);
     if (c->dpp_zeta.upload(zeta) || c->dpp_oo2z.upload(oo2z) || c->dpp_Px.upload(Px) || c->dpp_Py.upload(Py) || c->dpp_Pz.upload(Pz) ||
-        c->dpp_K.upload(K) || c->dpp_cut.alloc(zeta.size()) || c->dap_qa.upload(c->ap_qa) ||
+        c->dpp_K.upload(K) || c->dpp_ea.upload(ea) || c->dpp_eb.upload(eb) || c->dpp_cut.alloc(zeta.size()) ||
+        c->dap_qa.upload(c->ap_qa) ||
         c->dap_qb.upload(c->ap_qb) || c->dap_prim0.upload(c->ap_prim0) || c->dap_nprim.upload(c->ap_nprim))
         return -1;
     return 0;
@@ -608,7 +573,63 @@ extern "C" int qk_set_schwarz(qk_ctx* c, const double* Ycutoff, const double* cu
 }
 
 // ---------------------------------------------------------------------------
-// pair lists and the per-class work queue (host, once per geometry/options)
+// pair lists and the per-class work queue (once per geometry/options)
+
+using ListSumsFn = std::function<int(const std::vector<long long>&, std::vector<ListDst>&)>;
+
+// The work lists of the entries E (qk_lists.hip): per-item counts and their
+// scans on the device, the per-class sums to the host, where dst_f allocates
+// the lists and says where they go; then the lists are written.  With runs_f
+// (the Fock lists) the per-bra runs follow in the same way.  Synchronous.
+static int build_lists(qk_ctx* c, const std::vector<ListEntry>& E, const std::vector<KetGroup>& KG,
+                       const std::vector<ListClass>& LC, const std::vector<long long>& cls_item,
+                       const std::vector<long long>& cls_bra, double tau, double thr, const ListSumsFn& dst_f,
+                       const ListSumsFn* runs_f) {
+    const int ncls = kNumQuartetClasses;
+    const long long N = cls_item[ncls], NB = cls_bra[ncls];
+    std::vector<ListDst> dst(ncls, ListDst{});
+    std::vector<long long> sums((size_t)ncls * kListCounts, 0), rsums((size_t)ncls * kRunCounts, 0);
+    if (N == 0 || E.empty()) {
+        if (dst_f(sums, dst)) return -1;
+        return runs_f ? (*runs_f)(rsums, dst) : 0;
+    }
+    if (N + 1 > INT32_MAX || NB + 1 > INT32_MAX) { set_err("more than 2^31 work-list items"); return -1; }
+    if (c->dl_E.upload(E) || c->dl_KG.upload(KG) || c->dl_LC.upload(LC) || c->dl_ci.upload(cls_item) ||
+        c->dl_cb.upload(cls_bra) || c->dl_items.alloc((size_t)N) || c->dl_cnt.alloc((size_t)kListCounts * (N + 1)) ||
+        c->dl_off.alloc((size_t)kListCounts * (N + 1)) || c->dl_sums.alloc(sums.size()))
+        return -1;
+    const size_t tb = std::max(list_scan_bytes(N + 1), list_scan_bytes(NB + 1));
+    if (c->dl_tmp.alloc(std::max<size_t>(tb, 1))) return -1;
+    const int ne = (int)E.size();
+    if (lists_count(c->dl_E.p, ne, c->dl_KG.p, c->dl_LC.p, c->dp_Y.p, tau, thr, N, c->dl_items.p, c->dl_cnt.p,
+                    c->dl_off.p, c->dl_tmp.p, tb, c->dl_ci.p, ncls, c->dl_sums.p, nullptr)) {
+        set_err("work-list count pass failed: %s", hipGetErrorString(hipGetLastError()));
+        return -1;
+    }
+    HIPCHK(hipMemcpy(sums.data(), c->dl_sums.p, sums.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    if (dst_f(sums, dst) || c->dl_dst.upload(dst)) return -1;
+    lists_write(c->dl_E.p, ne, c->dl_KG.p, c->dl_LC.p, N, c->dl_items.p, c->dl_off.p, c->dl_ci.p, c->dl_dst.p, nullptr);
+    HIPCHK(hipGetLastError());
+    if (runs_f) {
+        if (c->dl_rcnt.alloc((size_t)kRunCounts * (NB + 1)) || c->dl_roff.alloc((size_t)kRunCounts * (NB + 1)) ||
+            c->dl_rsums.alloc(rsums.size()))
+            return -1;
+        if (lists_runs_count(c->dl_E.p, ne, c->dl_LC.p, NB, c->dl_items.p, c->dl_rcnt.p, c->dl_roff.p, c->dl_tmp.p, tb,
+                             c->dl_cb.p, ncls, c->dl_rsums.p, nullptr)) {
+            set_err("work-list run pass failed: %s", hipGetErrorString(hipGetLastError()));
+            return -1;
+        }
+        HIPCHK(hipMemcpy(rsums.data(), c->dl_rsums.p, rsums.size() * sizeof(long long), hipMemcpyDeviceToHost));
+        for (long long v : rsums)
+            if (v > INT32_MAX) { set_err("more than 2^31 runs in a class"); return -1; }
+        if ((*runs_f)(rsums, dst) || c->dl_dst.upload(dst)) return -1;
+        lists_runs_write(c->dl_E.p, ne, c->dl_LC.p, NB, c->dl_items.p, c->dl_off.p, N, c->dl_ci.p, c->dl_roff.p,
+                         c->dl_cb.p, c->dl_dst.p, nullptr);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(nullptr));
+    return 0;
+}
 
 static int prepare(qk_ctx* c, const qk_opts* o) {
     if (!c->have_system || !c->have_schwarz) {
@@ -663,24 +684,18 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         psh[k] = c->q_shell[qa[k]] | (c->q_shell[qb[k]] << 16);
         pat[k] = c->q_atom[qa[k]] | (c->q_atom[qb[k]] << 16);
     }
-    // primitive-major copy per group: entry (pair k, primitive j) at
-    // base(group) + j * count(group) + (k - start(group))
-    std::vector<double> kz, ko, kx, ky, kzz, kK, kc, kea, keb;
+    // primitive-major layout per group: entry (pair k, primitive j) at
+    // base(group) + j * count(group) + (k - start(group)); the copy itself runs
+    // on the device (gather_prims_kernel)
+    long long nprim_total = 0;
     for (const Group& g : groups) {
-        const int base = (int)kz.size();
         const int np = c->ap_nprim[sel[g.start]];
-        for (int j = 0; j < np; ++j)
-            for (int k = g.start; k < g.start + g.count; ++k) {
-                const int src = c->ap_prim0[sel[k]] + j;
-                kz.push_back(c->ap_zeta[src]); ko.push_back(c->ap_oo2z[src]);
-                kx.push_back(c->ap_Px[src]); ky.push_back(c->ap_Py[src]); kzz.push_back(c->ap_Pz[src]);
-                kK.push_back(c->ap_K[src]); kc.push_back(c->ap_cut[src]);
-                kea.push_back(c->ap_ea[src]); keb.push_back(c->ap_eb[src]);
-            }
         for (int k = g.start; k < g.start + g.count; ++k) {
-            prim0[k] = base + (k - g.start);
+            prim0[k] = (int)(nprim_total + (k - g.start));
             pstride[k] = g.count;
         }
+        nprim_total += (long long)np * g.count;
+        if (nprim_total > INT32_MAX) { set_err("more than 2^31 primitive pairs"); return -1; }
     }
     // the pair-indexed Coulomb buffer: per group, element-major blocks
     std::vector<int> jofs(npair);
@@ -691,94 +706,31 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         jsize += (long long)ne * g.count;
         if (jsize > INT32_MAX) { set_err("pair Coulomb buffer beyond 2^31 entries"); return -1; }
     }
-
-    // per-class tiles: for every (bra group, ket group) block and every bra,
-    // the prefix of kets passing Y_ij*Y_kl > tau cut into runs of <= 64
-    c->classes.clear();
-    c->classes.resize(kNumQuartetClasses);
-    std::vector<char> too_big(kNumQuartetClasses, 0);
-    for_each_class([&](int k) {
-        const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
-        const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
-        ClassHost& CH = c->classes[k];
-        long long total = 0, total32 = 0, seg_off = 0, seg_off32 = 0;
-        // Runs (one bra pair per block) of at most 2048 kets where one strong
-        // bra pair's kets would otherwise hold a single block for much of the
-        // kernel: (ss|ss), and the FP64 list of a mixed build (the strongest
-        // pairs: few bras, long ket ranges).  Measured on (H2O)32: (ss|ss) FP64
-        // 3.40 -> 2.49 ms, FP32 1.41 -> 1.22 ms; mixed FP64 lists 0.4 ms less in
-        // total (psps 0.61 -> 0.50 ms).  Elsewhere the cap was slower (psss FP64
-        // 4.40 -> 5.15 ms): every run zeroes and adds its exchange rows.
-        const bool cap_ss = kClassL[k][0] + kClassL[k][1] == 0;
-        const int run_cap = cap_ss || mixed ? 2048 : INT_MAX;
-        const int run_cap32 = cap_ss ? 2048 : INT_MAX;
-        for (size_t gb = 0; gb < groups.size(); ++gb) {
-            if (groups[gb].cls != cb) continue;
-            const Group& G1 = groups[gb];
-            for (int i = 0; i < G1.count; ++i) {
-                const double yp = Yp[G1.start + i];
-                for (size_t gk = 0; gk < groups.size(); ++gk) {
-                    if (groups[gk].cls != ck) continue;
-                    if (cb == ck && gk < gb) continue;
-                    const bool same = (gb == gk);
-                    const Group& G2 = groups[gk];
-                    // kets are sorted by Y descending: first one failing Y_ij*Y_kl > tau
-                    int lo = 0, hi = G2.count;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (yp * Yp[G2.start + mid] > o->tau) lo = mid + 1; else hi = mid;
-                    }
-                    const int q_begin = same ? i : 0;
-                    // mixed precision: kets with Y_ij*Y_kl >= threshold stay FP64,
-                    // the rest of the prefix goes to the FP32 list
-                    int lo64 = lo;
-                    if (mixed && fock_class_has_f32(k)) {
-                        int a0 = 0, a1 = lo;
-                        while (a0 < a1) {
-                            const int mid = (a0 + a1) >> 1;
-                            if (yp * Yp[G2.start + mid] >= o->fp32_threshold) a0 = mid + 1; else a1 = mid;
-                        }
-                        lo64 = std::max(a0, q_begin);
-                    }
-                    // a wavefront of the cooperative kernel works through its tile's
-                    // quartets one after another: short tiles spread them over the GPU
-                    const int chunk = fock_class_is_team(k) ? 2 : 64;
-                    for (int q = q_begin; q < lo64; q += chunk)
-                        CH.tiles.push_back({G1.start + i, G2.start + q, std::min(chunk, lo64 - q), 0});
-                    for (int q = lo64; q < lo; q += chunk)
-                        CH.tiles32.push_back({G1.start + i, G2.start + q, std::min(chunk, lo - q), 0});
-                    if (!fock_class_is_team(k)) {
-                        if (lo64 > q_begin)
-                            push_segment(CH.segs, G1.start + i, G2.start + q_begin, G2.start + lo64, run_cap, seg_off);
-                        if (lo > lo64)
-                            push_segment(CH.segs32, G1.start + i, G2.start + lo64, G2.start + lo, run_cap32, seg_off32);
-                    }
-                    if (lo > q_begin) total += lo - q_begin;
-                    if (lo > lo64) total32 += lo - lo64;
-                }
-            }
-        }
-        tile_runs(CH.tiles, CH.runs, run_cap);
-        tile_runs(CH.tiles32, CH.runs32, run_cap32);
-        seg_runs(CH.segs, CH.segruns, run_cap);
-        seg_runs(CH.segs32, CH.segruns32, run_cap32);
-        too_big[k] = seg_off > INT32_MAX || seg_off32 > INT32_MAX;
-        CH.total = total;
-        CH.total32 = total32;
-    });
-    for (int k = 0; k < kNumQuartetClasses; ++k)
-        if (too_big[k]) {
-            set_err("class %d: more than 2^31 candidate quartets", k);
-            return -1;
-        }
-
     (void)hipSetDevice(c->device);
     if (c->dp_qa.upload(qa) || c->dp_qb.upload(qb) || c->dp_prim0.upload(prim0) || c->dp_pstride.upload(pstride) ||
-        c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dk_zeta.upload(kz) || c->dk_oo2z.upload(ko) ||
-        c->dk_Px.upload(kx) || c->dk_Py.upload(ky) || c->dk_Pz.upload(kzz) || c->dk_K.upload(kK) ||
-        c->dk_cut.upload(kc) || c->dk_ea.upload(kea) || c->dk_eb.upload(keb) || c->dp_sh.upload(psh) ||
-        c->dp_at.upload(pat) || c->dp_jofs.upload(jofs) || c->d_J.alloc((size_t)std::max(jsize, 1LL)))
+        c->dp_nprim.upload(nprim) || c->dp_Y.upload(Yp) || c->dp_sh.upload(psh) || c->dp_at.upload(pat) ||
+        c->dp_jofs.upload(jofs) || c->d_sel.upload(sel) || c->d_J.alloc((size_t)std::max(jsize, 1LL)))
         return -1;
+    {
+        const size_t np = (size_t)std::max(nprim_total, 1LL);
+        DBuf<double>* dd[kPrimArrays] = {&c->dk_zeta, &c->dk_oo2z, &c->dk_Px, &c->dk_Py, &c->dk_Pz,
+                                         &c->dk_K, &c->dk_cut, &c->dk_ea, &c->dk_eb};
+        DBuf<float>* df[kPrimArrays] = {&c->dkf_zeta, &c->dkf_oo2z, &c->dkf_Px, &c->dkf_Py, &c->dkf_Pz,
+                                        &c->dkf_K, &c->dkf_cut, &c->dkf_ea, &c->dkf_eb};
+        DBuf<double>* src[kPrimArrays] = {&c->dpp_zeta, &c->dpp_oo2z, &c->dpp_Px, &c->dpp_Py, &c->dpp_Pz,
+                                          &c->dpp_K, &c->dpp_cut, &c->dpp_ea, &c->dpp_eb};
+        PrimArrays ps{}, pd{};
+        PrimArraysF pf{};
+        for (int a = 0; a < kPrimArrays; ++a) {
+            if (dd[a]->alloc(np) || (mixed && df[a]->alloc(np))) return -1;
+            ps.a[a] = src[a]->p;
+            pd.a[a] = dd[a]->p;
+            pf.a[a] = mixed ? df[a]->p : nullptr;
+        }
+        launch_gather_prims(c->d_sel.p, c->dap_prim0.p, c->dp_nprim.p, c->dp_prim0.p, c->dp_pstride.p, npair, ps, pd,
+                            pf, nullptr);
+        HIPCHK(hipGetLastError());
+    }
     c->jsize = jsize;
     c->grp_cls.clear(); c->grp_start.clear(); c->grp_count.clear();
     for (const Group& g : groups) {
@@ -788,39 +740,100 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
     }
     c->pair_Y = Yp;
     c->grad_prepared = false;
-    for (auto& CH : c->classes) {
-        if (CH.d_tiles.upload(CH.tiles) || CH.d_tiles32.upload(CH.tiles32) || CH.d_runs.upload(CH.runs) ||
-            CH.d_runs32.upload(CH.runs32))
-            return -1;
-        size_t nl = 0, nl32 = 0;
-        for (const int4& g : CH.segs) nl += (size_t)(g.z - g.y);
-        for (const int4& g : CH.segs32) nl32 += (size_t)(g.z - g.y);
+
+    // per-class work lists on the device: for every (bra group, bra, ket
+    // group) the prefix of kets passing Y_ij*Y_kl > tau, in tiles of <= 64
+    // kets (2 for the cooperative kernel, whose wavefront works through its
+    // tile's quartets one after another), compaction segments and runs.
+    // Runs (one bra pair per block) hold at most 2048 kets where one strong bra
+    // pair's kets would otherwise hold a single block for much of the kernel:
+    // (ss|ss), and the FP64 list of a mixed build (the strongest pairs: few
+    // bras, long ket ranges).  Measured on (H2O)32: (ss|ss) FP64 3.40 -> 2.49 ms,
+    // FP32 1.41 -> 1.22 ms; mixed FP64 lists 0.4 ms less in total (psps 0.61 ->
+    // 0.50 ms).  Elsewhere the cap was slower (psss FP64 4.40 -> 5.15 ms): every
+    // run zeroes and adds its exchange rows.
+    std::vector<ListEntry> E;
+    std::vector<KetGroup> KG;
+    std::vector<ListClass> LC(kNumQuartetClasses);
+    std::vector<long long> cls_item(kNumQuartetClasses + 1), cls_bra(kNumQuartetClasses + 1);
+    long long nitem = 0, nbra = 0;
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        cls_item[k] = nitem;
+        cls_bra[k] = nbra;
+        const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
+        const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
+        const bool team = fock_class_is_team(k);
+        const bool cap_ss = kClassL[k][0] + kClassL[k][1] == 0;
+        LC[k] = ListClass{team ? 2 : 64, cap_ss || mixed ? 2048 : INT_MAX, cap_ss ? 2048 : INT_MAX,
+                          mixed && fock_class_has_f32(k) ? 1 : 0, team ? 0 : 1, 0, team ? 0 : 1, 0};
+        for (size_t gb = 0; gb < groups.size(); ++gb) {
+            if (groups[gb].cls != cb) continue;
+            const int kg0 = (int)KG.size();
+            for (size_t gk = 0; gk < groups.size(); ++gk) {
+                if (groups[gk].cls != ck || (cb == ck && gk < gb)) continue;
+                KG.push_back(KetGroup{groups[gk].start, groups[gk].count, gb == gk ? 1 : 0, 0});
+            }
+            const int m = (int)KG.size() - kg0;
+            if (m == 0) continue;
+            E.push_back(ListEntry{groups[gb].start, groups[gb].count, m, kg0, nitem, nbra, k, 0});
+            nitem += (long long)groups[gb].count * m;
+            nbra += groups[gb].count;
+        }
+    }
+    cls_item[kNumQuartetClasses] = nitem;
+    cls_bra[kNumQuartetClasses] = nbra;
+    if (c->classes.size() != (size_t)kNumQuartetClasses) c->classes.resize(kNumQuartetClasses);
+    const ListSumsFn fock_dst = [&](const std::vector<long long>& sm, std::vector<ListDst>& dst) -> int {
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            ClassHost& CH = c->classes[k];
+            const long long* v = &sm[(size_t)k * kListCounts];
+            if (v[0] > INT32_MAX || v[1] > INT32_MAX || v[4] > INT32_MAX || v[5] > INT32_MAX) {
+                set_err("class %d: more than 2^31 tiles or candidate quartets", k);
+                return -1;
+            }
+            CH.ntiles = v[0]; CH.ntiles32 = v[1]; CH.nsegs = v[2]; CH.nsegs32 = v[3];
+            CH.total = v[4] + v[5];
+            CH.total32 = v[5];
+            if (CH.d_tiles.alloc(v[0]) || CH.d_tiles32.alloc(v[1]) || CH.d_segs.alloc(v[2]) ||
+                CH.d_segs32.alloc(v[3]) || CH.d_list.alloc(v[4]) || CH.d_list32.alloc(v[5]) ||
+                CH.d_counts.alloc(v[2]) || CH.d_counts32.alloc(v[3]))
+                return -1;
+            dst[k] = ListDst{CH.d_tiles.p, CH.d_tiles32.p, CH.d_segs.p, CH.d_segs32.p, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr};
+        }
+        return 0;
+    };
+    const ListSumsFn fock_runs = [&](const std::vector<long long>& rs, std::vector<ListDst>& dst) -> int {
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            ClassHost& CH = c->classes[k];
+            const long long* v = &rs[(size_t)k * kRunCounts];
+            CH.nruns = v[0]; CH.nruns32 = v[1]; CH.nsegruns = v[2]; CH.nsegruns32 = v[3];
+            if (CH.d_runs.alloc(v[0]) || CH.d_runs32.alloc(v[1]) || CH.d_segruns.alloc(v[2]) ||
+                CH.d_segruns32.alloc(v[3]))
+                return -1;
+            dst[k].r64 = CH.d_runs.p; dst[k].r32 = CH.d_runs32.p;
+            dst[k].sr64 = CH.d_segruns.p; dst[k].sr32 = CH.d_segruns32.p;
+        }
+        return 0;
+    };
+    if (build_lists(c, E, KG, LC, cls_item, cls_bra, o->tau, mixed ? o->fp32_threshold : 0.0, fock_dst, &fock_runs))
+        return -1;
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
         // compaction costs about as much per candidate as an FP32 (ss|ss), (ps|ss)
         // or (ps|ps) quartet saves, and more than an FP64 (ss|ss) one (measured on
         // (H2O)32); it pays for every other class
-        const int k = (int)(&CH - c->classes.data());
+        ClassHost& CH = c->classes[k];
         const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
         const bool small = kClassL[k][1] == 0 && kClassL[k][3] == 0 && LT <= 2 && kClassL[k][0] <= 1;
-        CH.compact = !CH.segs.empty() && LT >= 1;
-        CH.compact32 = !CH.segs32.empty() && !small;
-        if (CH.d_segs.upload(CH.segs) || CH.d_segs32.upload(CH.segs32) || CH.d_segruns.upload(CH.segruns) ||
-            CH.d_segruns32.upload(CH.segruns32) || CH.d_list.alloc(nl) || CH.d_list32.alloc(nl32) ||
-            CH.d_counts.alloc(CH.segs.size()) || CH.d_counts32.alloc(CH.segs32.size()))
-            return -1;
+        CH.compact = CH.nsegs > 0 && LT >= 1;
+        CH.compact32 = CH.nsegs32 > 0 && !small;
     }
-    if (mixed) {
-        auto f32 = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
-        if (c->dkf_zeta.upload(f32(kz)) || c->dkf_oo2z.upload(f32(ko)) || c->dkf_Px.upload(f32(kx)) ||
-            c->dkf_Py.upload(f32(ky)) || c->dkf_Pz.upload(f32(kzz)) || c->dkf_K.upload(f32(kK)) ||
-            c->dkf_cut.upload(f32(kc)) || c->dkf_ea.upload(f32(kea)) || c->dkf_eb.upload(f32(keb)) ||
-            c->d_Dpf.alloc((size_t)c->nbasis * c->nbasis))
-            return -1;
-        if (!c->d_boys_tab.p) {
-            std::vector<float> tab;
-            boys_table(tab);
-            if (c->d_boys_tab.upload(tab)) return -1;
-        }
+    if (mixed && !c->d_boys_tab.p) {
+        std::vector<float> tab;
+        boys_table(tab);
+        if (c->d_boys_tab.upload(tab)) return -1;
     }
+    if (mixed && c->d_Dpf.alloc((size_t)c->nbasis * c->nbasis)) return -1;
     c->nclass_pairs = npair;
     c->prepared = true;
     c->prep_tau = o->tau;
@@ -948,11 +961,11 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         HIPCHK(hipEventRecord(c->ev_start, s));
         HIPCHK(hipStreamWaitEvent(s64, c->ev_start, 0));
     }
-    // All classes run one after another on the caller's stream (one hardware
-    // queue: kernels that spill to scratch never run concurrently with others).
-    // Kernels that spill get a grid small enough that their private-segment
-    // backing stays under kScratchBudget (scratch_grid_cap); the grid-stride
-    // loop over tiles covers the rest.
+    // The classes of one list run one after another on their stream (the
+    // caller's; in a mixed build the FP64 lists on the second one), so at most
+    // two kernels are in flight.  Kernels that spill get a grid small enough
+    // that their private-segment backing stays under kScratchBudget
+    // (scratch_grid_cap); the grid-stride loop over tiles covers the rest.
     // LDS for one bra's exchange rows.  gfx950 gives a block up to 160 KB of
     // dynamic LDS (152 KB leave room for the bra slab and the kernels' static
     // LDS, <= 6 KB), but rows beyond 64 KB hold a CU to one or two blocks and
@@ -965,7 +978,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     constexpr size_t kRowsLdsMax = 152 * 1024, kRowsLdsSmall = 64 * 1024;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
-        if (CH.tiles.empty() && CH.tiles32.empty()) continue;
+        if (CH.ntiles == 0 && CH.ntiles32 == 0) continue;
         // one bra pair per block with its exchange rows in LDS, when they fit
         const int nrow = ncart_of(kClassL[k][0]) + ncart_of(kClassL[k][1]);
         const size_t rows_lds = sizeof(unsigned long long) * (size_t)nrow * n;
@@ -978,28 +991,28 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         const size_t slab64 = 7 * 64 * sizeof(double), slab32 = 7 * 64 * sizeof(float);
         const size_t lds64 = use_runs ? rows_lds + slab64 : 4 * slab64;
         const size_t lds32 = use_runs32 ? rows_lds + slab32 : 4 * slab32;
-        if (!CH.tiles.empty()) {
+        if (CH.ntiles) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s64));
             a.counters = fb->counters + 2 * k;
             if (fock_class_is_team(k)) {
-                ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
+                ClassWork w{CH.d_tiles.p, CH.ntiles, nullptr, 0};
                 // one 64-lane block per tile; up to 16 resident per CU
-                const int grid = (int)std::min<long long>((long long)CH.tiles.size(), 256LL * 16);
+                const int grid = (int)std::min<long long>(CH.ntiles, 256LL * 16);
                 launch_team_class(k, a, w, grid, s64);
             } else {
                 const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch(k), 4));
                 if (use_runs && CH.compact && fb->compact) {
                     // density screen first: every lane of the class kernel then has a quartet
-                    launch_compact(ca, CH.d_segs.p, (long long)CH.segs.size(), CH.d_list.p, CH.d_counts.p, s64);
-                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_segruns.p,
-                                (long long)CH.segruns.size(), CH.d_segs.p, CH.d_list.p, CH.d_counts.p};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.segruns.size(), cap), lds64, s64);
+                    launch_compact(ca, CH.d_segs.p, CH.nsegs, CH.d_list.p, CH.d_counts.p, s64);
+                    ClassWork w{CH.d_tiles.p, CH.ntiles, CH.d_segruns.p, CH.nsegruns, CH.d_segs.p, CH.d_list.p,
+                                CH.d_counts.p};
+                    launch_fock_class(k, a, w, (int)std::min<long long>(CH.nsegruns, cap), lds64, s64);
                 } else if (use_runs) {
-                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), CH.d_runs.p, (long long)CH.runs.size()};
-                    launch_fock_class(k, a, w, (int)std::min<long long>((long long)CH.runs.size(), cap), lds64, s64);
+                    ClassWork w{CH.d_tiles.p, CH.ntiles, CH.d_runs.p, CH.nruns};
+                    launch_fock_class(k, a, w, (int)std::min<long long>(CH.nruns, cap), lds64, s64);
                 } else {
-                    ClassWork w{CH.d_tiles.p, (long long)CH.tiles.size(), nullptr, 0};
-                    const long long want = ((long long)CH.tiles.size() + 3) / 4;   // 4 wavefronts per block
+                    ClassWork w{CH.d_tiles.p, CH.ntiles, nullptr, 0};
+                    const long long want = (CH.ntiles + 3) / 4;   // 4 wavefronts per block
                     launch_fock_class(k, a, w, (int)std::min<long long>(want, cap), lds64, s64);
                 }
             }
@@ -1007,20 +1020,20 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[k], s64));
             c->cls_timed[k] = c->timing;
         }
-        if (mixed && !CH.tiles32.empty()) {
+        if (mixed && CH.ntiles32) {
             a.counters = fb->counters + 2 * kNumQuartetClasses + 2 * k;
             const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch_f32(k), 4));
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
-            ClassWork w{CH.d_tiles32.p, (long long)CH.tiles32.size(), nullptr, 0};
-            int grid = (int)std::min<long long>(((long long)CH.tiles32.size() + 3) / 4, cap);
+            ClassWork w{CH.d_tiles32.p, CH.ntiles32, nullptr, 0};
+            int grid = (int)std::min<long long>((CH.ntiles32 + 3) / 4, cap);
             size_t lds = lds32;
             // one 64-lane block per tile; up to 20 resident per CU (five wavefronts per SIMD)
-            if (fock_class_is_team(k)) grid = (int)std::min<long long>((long long)CH.tiles32.size(), 256LL * 20);
+            if (fock_class_is_team(k)) grid = (int)std::min<long long>(CH.ntiles32, 256LL * 20);
             if (use_runs32 && CH.compact32 && fb->compact) {
-                launch_compact(ca, CH.d_segs32.p, (long long)CH.segs32.size(), CH.d_list32.p, CH.d_counts32.p, s);
+                launch_compact(ca, CH.d_segs32.p, CH.nsegs32, CH.d_list32.p, CH.d_counts32.p, s);
                 w.runs = CH.d_segruns32.p;
-                w.nruns = (long long)CH.segruns32.size();
+                w.nruns = CH.nsegruns32;
                 w.segs = CH.d_segs32.p;
                 w.list = CH.d_list32.p;
                 w.counts = CH.d_counts32.p;
@@ -1028,7 +1041,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 lds = lds32;
             } else if (use_runs32) {
                 w.runs = CH.d_runs32.p;
-                w.nruns = (long long)CH.runs32.size();
+                w.nruns = CH.nruns32;
                 grid = (int)std::min<long long>(w.nruns, cap);
                 lds = lds32;
             }
@@ -1278,8 +1291,8 @@ extern "C" int qk_class_info(qk_ctx* c, int k, int* l, long long* work) {
 
 // lanes occupied by the tiles of class k: quartets / (64 * tiles)
 extern "C" double qk_class_lane_fill(qk_ctx* c, int k) {
-    if (!c || !c->prepared || k < 0 || k >= kNumQuartetClasses || c->classes[k].tiles.empty()) return 0.0;
-    return double(c->classes[k].total) / (64.0 * double(c->classes[k].tiles.size()));
+    if (!c || !c->prepared || k < 0 || k >= kNumQuartetClasses || c->classes[k].ntiles == 0) return 0.0;
+    return double(c->classes[k].total) / (64.0 * double(c->classes[k].ntiles));
 }
 
 extern "C" int qk_compute_schwarz(qk_ctx* c, double* Ycut_out, double* cutprim_out) {
@@ -1358,94 +1371,63 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
             std::vector<int> blob(kGradBlobs[k], kGradBlobs[k] + kGradBlobLen[k]);
             if (c->d_gtab[k].upload(blob)) return -1;
         }
-    // the work lists of every class on host threads, then the uploads here
-    std::vector<long long> goff(2 * kNumQuartetClasses, 0);
-    std::vector<char> too_big(kNumQuartetClasses, 0);
-    for_each_class([&](int k) {
+    // the work lists on the device: (bra group, ket group, bra) order, so that
+    // neighbouring wavefronts read the same kets (measured faster than
+    // bra-major order for the gradient); light classes one quartet per lane in
+    // tiles of up to 64 kets, the table-driven kernel one quartet at a time per
+    // wavefront in tiles of 16, 4 or 1
+    std::vector<ListEntry> E;
+    std::vector<KetGroup> KG;
+    std::vector<ListClass> LC(kNumQuartetClasses);
+    std::vector<long long> cls_item(kNumQuartetClasses + 1), cls_bra(kNumQuartetClasses + 1, 0);
+    long long nitem = 0;
+    const size_t ng = c->grp_cls.size();
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        cls_item[k] = nitem;
         const int cb = pair_class_of(kClassL[k][0], kClassL[k][1]);
         const int ck = pair_class_of(kClassL[k][2], kClassL[k][3]);
         const int LT = kClassL[k][0] + kClassL[k][1] + kClassL[k][2] + kClassL[k][3];
-        // light classes: one quartet per lane, runs of up to 64 kets; the
-        // table-driven kernel: one quartet at a time per wavefront
-        const int chunk = grad_class_is_light(k) ? 64 : (LT <= 3 ? 16 : (LT <= 5 ? 4 : 1));
-        std::vector<Tile>& T = c->gtiles[k];
-        std::vector<Tile>& T32 = c->gtiles32[k];
-        T.clear();
-        T32.clear();
         const bool light = grad_class_is_light(k);
-        long long off[2] = {0, 0};
-        for (int f = 0; f < 2; ++f) {
-            c->gsegs[f][k].clear();
-            c->gslot0[f][k].clear();
-        }
-        const size_t ng = c->grp_cls.size();
-        // (bra group, ket group, bra) order: neighbouring wavefronts read the
-        // same kets (measured faster than bra-major order for the gradient)
+        const int chunk = light ? 64 : (LT <= 3 ? 16 : (LT <= 5 ? 4 : 1));
+        LC[k] = ListClass{chunk, INT_MAX, INT_MAX, mixed ? 1 : 0, light ? 1 : 0, 1, 0, 0};
         for (size_t gb = 0; gb < ng; ++gb) {
             if (c->grp_cls[gb] != cb) continue;
             for (size_t gk = 0; gk < ng; ++gk) {
-                if (c->grp_cls[gk] != ck) continue;
-                if (cb == ck && gk < gb) continue;
-                const bool same = gb == gk;
-                const int s1 = c->grp_start[gb], n1 = c->grp_count[gb];
-                const int s2 = c->grp_start[gk], n2 = c->grp_count[gk];
-                for (int i = 0; i < n1; ++i) {
-                    const double yp = c->pair_Y[s1 + i];
-                    int lo = 0, hi = n2;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (yp * c->pair_Y[s2 + mid] > cutoff) lo = mid + 1; else hi = mid;
-                    }
-                    const int q_begin = same ? i : 0;
-                    int lo64 = lo;
-                    if (mixed) {
-                        int a0 = 0, a1 = lo;
-                        while (a0 < a1) {
-                            const int mid = (a0 + a1) >> 1;
-                            if (yp * c->pair_Y[s2 + mid] >= o->fp32_threshold) a0 = mid + 1; else a1 = mid;
-                        }
-                        lo64 = std::max(a0, q_begin);
-                    }
-                    // a light class's segment owns the slots of its tiles (64 kets each)
-                    if (light && lo64 > q_begin) {
-                        c->gsegs[0][k].push_back({s1 + i, s2 + q_begin, s2 + lo64, (int)off[0]});
-                        c->gslot0[0][k].push_back((int)T.size());
-                        off[0] += lo64 - q_begin;
-                    }
-                    if (light && lo > lo64) {
-                        c->gsegs[1][k].push_back({s1 + i, s2 + lo64, s2 + lo, (int)off[1]});
-                        c->gslot0[1][k].push_back((int)T32.size());
-                        off[1] += lo - lo64;
-                    }
-                    for (int q = q_begin; q < lo64; q += chunk)
-                        T.push_back({s1 + i, s2 + q, std::min(chunk, lo64 - q), 0});
-                    for (int q = lo64; q < lo; q += chunk)
-                        T32.push_back({s1 + i, s2 + q, std::min(chunk, lo - q), 0});
-                }
+                if (c->grp_cls[gk] != ck || (cb == ck && gk < gb)) continue;
+                KG.push_back(KetGroup{c->grp_start[gk], c->grp_count[gk], gb == gk ? 1 : 0, 0});
+                E.push_back(ListEntry{c->grp_start[gb], c->grp_count[gb], 1, (int)KG.size() - 1, nitem, nitem, k, 0});
+                nitem += c->grp_count[gb];
             }
         }
-        too_big[k] = off[0] > INT32_MAX || off[1] > INT32_MAX || T.size() > INT32_MAX || T32.size() > INT32_MAX;
-        goff[2 * k] = off[0];
-        goff[2 * k + 1] = off[1];
-    });
-    for (int k = 0; k < kNumQuartetClasses; ++k) {
-        if (too_big[k]) {
-            set_err("gradient class %d: more than 2^31 candidate quartets", k);
-            return -1;
-        }
-        const std::vector<Tile>& T = c->gtiles[k];
-        const std::vector<Tile>& T32 = c->gtiles32[k];
-        if (c->d_gtiles[k].upload(T) || c->d_gtiles32[k].upload(T32)) return -1;
-        const bool light = grad_class_is_light(k);
-        for (int f = 0; f < 2; ++f) {
-            const size_t nslot = light ? (f ? T32.size() : T.size()) : 0;
-            if (c->d_gsegs[f][k].upload(c->gsegs[f][k]) || c->d_gslot0[f][k].upload(c->gslot0[f][k]) ||
-                c->d_gsteps[f][k].alloc(std::max<size_t>(nslot, 1)) ||
-                c->d_glist[f][k].alloc((size_t)std::max(goff[2 * k + f], 1LL)) ||
-                c->d_gcounts[f][k].alloc(std::max<size_t>(c->gsegs[f][k].size(), 1)))
-                return -1;
-        }
     }
+    cls_item[kNumQuartetClasses] = nitem;
+    const ListSumsFn grad_dst = [&](const std::vector<long long>& sm, std::vector<ListDst>& dst) -> int {
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            const long long* v = &sm[(size_t)k * kListCounts];
+            if (v[0] > INT32_MAX || v[1] > INT32_MAX || v[4] > INT32_MAX || v[5] > INT32_MAX) {
+                set_err("gradient class %d: more than 2^31 tiles or candidate quartets", k);
+                return -1;
+            }
+            const bool light = grad_class_is_light(k);
+            c->ngtiles[k] = v[0];
+            c->ngtiles32[k] = v[1];
+            if (c->d_gtiles[k].alloc(v[0]) || c->d_gtiles32[k].alloc(v[1])) return -1;
+            for (int f = 0; f < 2; ++f) {
+                c->ngsegs[f][k] = v[2 + f];
+                const long long nslot = light ? v[f] : 0;   // a light class's segment owns the slots of its tiles
+                if (c->d_gsegs[f][k].alloc(v[2 + f]) || c->d_gslot0[f][k].alloc(v[2 + f]) ||
+                    c->d_gsteps[f][k].alloc(std::max<long long>(nslot, 1)) ||
+                    c->d_glist[f][k].alloc(std::max<long long>(v[4 + f], 1)) ||
+                    c->d_gcounts[f][k].alloc(std::max<long long>(v[2 + f], 1)))
+                    return -1;
+            }
+            dst[k] = ListDst{c->d_gtiles[k].p, c->d_gtiles32[k].p, c->d_gsegs[0][k].p, c->d_gsegs[1][k].p,
+                             c->d_gslot0[0][k].p, c->d_gslot0[1][k].p, nullptr, nullptr, nullptr, nullptr};
+        }
+        return 0;
+    };
+    if (build_lists(c, E, KG, LC, cls_item, cls_bra, cutoff, mixed ? o->fp32_threshold : 0.0, grad_dst, nullptr))
+        return -1;
     if (c->d_gacc.alloc(3 * (size_t)c->natom) || c->d_gcounters.alloc(kNumGradCounters)) return -1;
     c->grad_prep_cutoff = cutoff;
     c->grad_prep_precision = o->precision;
@@ -1535,9 +1517,9 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         const bool fp32 = pass == 1;
         const hipStream_t ls = fp32 ? s : s64;
         for (int k = 0; k < kNumQuartetClasses; ++k) {
-            const std::vector<Tile>& T = fp32 ? c->gtiles32[k] : c->gtiles[k];
-            if (T.empty()) continue;
-            ClassWork w{(fp32 ? c->d_gtiles32[k] : c->d_gtiles[k]).p, (long long)T.size()};
+            const long long nt = fp32 ? c->ngtiles32[k] : c->ngtiles[k];
+            if (nt == 0) continue;
+            ClassWork w{(fp32 ? c->d_gtiles32[k] : c->d_gtiles[k]).p, nt};
             a.counters = c->d_gcounters.p + 2 * (fp32 ? kNumQuartetClasses + k : k);
             const int slot = (fp32 ? 3 : 2) * kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], ls));
@@ -1547,17 +1529,17 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 CompactArgs cq = ca;
                 cq.steps = c->d_gsteps[f][k].p;
                 cq.slot0 = c->d_gslot0[f][k].p;
-                launch_compact(cq, c->d_gsegs[f][k].p, (long long)c->gsegs[f][k].size(), c->d_glist[f][k].p,
+                launch_compact(cq, c->d_gsegs[f][k].p, c->ngsegs[f][k], c->d_glist[f][k].p,
                                c->d_gcounts[f][k].p, ls);
                 w.list = c->d_glist[f][k].p;
                 w.steps = cq.steps;
-                const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
+                const int grid = (int)std::min<long long>((nt + 3) / 4, 256LL * 8);
                 if (launch_grad_light(k, a, w, grid, fp32, ls)) {
                     set_err("gradient kernel launch failed for class %d", k);
                     return -1;
                 }
             } else if (grad_class_is_light(k)) {
-                const int grid = (int)std::min<long long>(((long long)T.size() + 3) / 4, 256LL * 8);
+                const int grid = (int)std::min<long long>((nt + 3) / 4, 256LL * 8);
                 if (launch_grad_light(k, a, w, grid, fp32, ls)) {
                     set_err("gradient kernel launch failed for class %d", k);
                     return -1;
@@ -1569,7 +1551,7 @@ static int grad_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 // dppp, measured slower at five (12.4 ms at four, 16.0 at five on
                 // (H2O)32): it runs the FP32 form built for four
                 const bool dppp = kClassL[k][0] == 2 && kClassL[k][1] == 1 && kClassL[k][2] == 1 && kClassL[k][3] == 1;
-                const int grid = (int)std::min<long long>((long long)T.size(), 256LL * (fp32 && !dppp ? 20 : 16));
+                const int grid = (int)std::min<long long>(nt, 256LL * (fp32 && !dppp ? 20 : 16));
                 if (launch_grad_class(a, w, kGradBlobs[k], c->d_gtab[k].p, lds, grid, fp32, dppp, ls)) {
                     set_err("gradient kernel launch failed for class %d (%zu bytes LDS)", k, lds);
                     return -1;

@@ -114,12 +114,55 @@ struct DevPrims {
 // q0 .. q0+len-1 of one ket group (one per lane).  Bra and ket groups are runs
 // of pairs of one pair class with equal primitive-pair count, sorted by Schwarz
 // bound descending, so the kets passing Y_p*Y_q > tau form a prefix of the ket
-// group for every bra; the host cuts those prefixes into tiles.
+// group for every bra; those prefixes are cut into tiles (qk_lists.hip).
 struct Tile {
     int p;
     int q0;
     int len;
     int pad;
+};
+
+// Work-list construction on the device (qk_lists.hip).  An entry is a block of
+// items of one quartet class: bra pairs s1 .. s1+n1-1 of one bra group, each
+// against the m ket groups kg[kg0 .. kg0+m) (item item0 + i*m + t: bra i, ket
+// group t).  The Fock lists have one entry per bra group, the gradient lists
+// one per (bra group, ket group) with m = 1.  bra0 numbers the entry's bras
+// across all entries (the Fock runs).
+struct ListEntry {
+    int s1, n1, m, kg0;
+    long long item0, bra0;
+    int cls, pad;
+};
+struct KetGroup {
+    int s2, n2, same, pad;   // first pair, count; same: the bra group itself (kets from the bra on)
+};
+struct ListClass {
+    int chunk;          // kets per tile
+    int cap64, cap32;   // candidates per run and per segment (FP64, FP32 lists)
+    int split;          // mixed precision: kets with Y_ij*Y_kl >= threshold stay FP64
+    int segs;           // compaction segments are listed
+    int grad;           // gradient form: one segment per item, with its first tile
+    int runs;           // runs of tiles and segments per bra (Fock)
+    int pad;
+};
+// per class: where the lists go (FP64 / FP32 tiles, segments, the gradient's
+// first tile per segment, runs of tiles and of segments)
+struct ListDst {
+    Tile *t64, *t32;
+    int4 *s64, *s32;
+    int *slot64, *slot32;
+    int2 *r64, *r32, *sr64, *sr32;
+};
+constexpr int kListCounts = 6;   // per item: tiles, tiles32, segs, segs32, candidates, candidates32
+constexpr int kRunCounts = 4;    // per bra: runs, runs32, segment runs, segment runs32
+// the primitive-pair fields copied primitive-major per group: zeta, 1/2zeta,
+// Px, Py, Pz, K, cut, and the two exponents
+constexpr int kPrimArrays = 9;
+struct PrimArrays {
+    double* a[kPrimArrays];
+};
+struct PrimArraysF {
+    float* a[kPrimArrays];   // a[0] == nullptr: no FP32 copy
 };
 
 struct ClassWork {
