@@ -108,6 +108,14 @@ struct DBuf {
     size_t cap = 0;   // elements allocated: a smaller count reuses the buffer
                       // (work lists are rebuilt per option set; hipFree and
                       // hipMalloc of hundreds of buffers each time cost more)
+    bool owned = true;   // false: a view into memory owned elsewhere (an arena)
+    // a view of count elements at q (not freed here)
+    void view(T* q, size_t count) {
+        release();
+        p = count ? q : nullptr;
+        n = count;
+        owned = false;
+    }
     int alloc(size_t count) {
         if (count == 0) { release(); return 0; }
         if (p && count <= cap) { n = count; return 0; }
@@ -129,11 +137,31 @@ struct DBuf {
         return 0;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = cap = 0;
+        owned = true;
     }
     ~DBuf() { release(); }
+};
+
+// Carves views out of one device allocation (sizes first, then place): the work
+// lists of all classes in one hipMalloc, reused while large enough.
+struct Arena {
+    DBuf<unsigned char> mem;
+    std::vector<size_t> off;
+    size_t total = 0;
+    void reset() { off.clear(); total = 0; }
+    // reserve count elements of T; returns the slot
+    template <typename T>
+    int reserve(size_t count) {
+        off.push_back(total);
+        total += (count * sizeof(T) + 255) / 256 * 256;
+        return (int)off.size() - 1;
+    }
+    int commit() { return mem.alloc(std::max<size_t>(total, 1)); }
+    template <typename T>
+    void place(DBuf<T>& b, int slot, size_t count) { b.view(reinterpret_cast<T*>(mem.p + off[slot]), count); }
 };
 
 // ---------------------------------------------------------------------------
@@ -270,6 +298,7 @@ struct qk_ctx {
     DBuf<int4> dl_items;
     DBuf<long long> dl_ci, dl_cb, dl_cnt, dl_off, dl_sums, dl_rcnt, dl_roff, dl_rsums;
     DBuf<unsigned char> dl_tmp;
+    Arena fock_arena, fock_run_arena, grad_arena;   // the lists themselves
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
@@ -660,11 +689,44 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
         if (!(y * Ymax > o->tau)) continue;  // no partner can pass Y_ij*Y_kl > tau
         sel.push_back((int)k);
     }
-    std::stable_sort(sel.begin(), sel.end(), [&](int x, int y) {
-        if (c->ap_cls[x] != c->ap_cls[y]) return c->ap_cls[x] < c->ap_cls[y];
-        if (c->ap_nprim[x] != c->ap_nprim[y]) return c->ap_nprim[x] > c->ap_nprim[y];
-        return c->ap_Y[x] > c->ap_Y[y];
-    });
+    // the order of a stable sort by (class, primitive-pair count descending, Y
+    // descending): a stable LSD radix sort on Y descending (the bits of a
+    // positive double order as the value; complemented for descending), then a
+    // stable counting sort by (class, count) bucket
+    {
+        const size_t m = sel.size();
+        std::vector<uint64_t> key(m), key2(m);
+        std::vector<int> idx2(m);
+        for (size_t t = 0; t < m; ++t) {
+            uint64_t b;
+            memcpy(&b, &c->ap_Y[sel[t]], sizeof b);
+            key[t] = ~b;
+        }
+        std::vector<int> hist(1 << 16);
+        for (int sh = 0; sh < 64; sh += 16) {
+            std::fill(hist.begin(), hist.end(), 0);
+            for (size_t t = 0; t < m; ++t) ++hist[(key[t] >> sh) & 0xffff];
+            if (m && hist[(key[0] >> sh) & 0xffff] == (int)m) continue;   // one digit value: order unchanged
+            int run = 0;
+            for (int& h : hist) { const int v = h; h = run; run += v; }
+            for (size_t t = 0; t < m; ++t) {
+                const int d = hist[(key[t] >> sh) & 0xffff]++;
+                key2[d] = key[t];
+                idx2[d] = sel[t];
+            }
+            key.swap(key2);
+            sel.swap(idx2);
+        }
+        constexpr int kNb = kNumPairClasses * 65;   // (class, 64 - primitive pairs)
+        std::vector<int> cnt(kNb + 1, 0);
+        for (int e : sel) {
+            if (c->ap_nprim[e] > 64) { set_err("more than 64 primitive pairs in a Qshell pair"); return -1; }
+            ++cnt[c->ap_cls[e] * 65 + (64 - c->ap_nprim[e]) + 1];
+        }
+        for (int b = 0; b < kNb; ++b) cnt[b + 1] += cnt[b];
+        for (int e : sel) idx2[cnt[c->ap_cls[e] * 65 + (64 - c->ap_nprim[e])]++] = e;
+        sel.swap(idx2);
+    }
     const int npair = (int)sel.size();
     struct Group { int cls, start, count; };
     std::vector<Group> groups;
@@ -794,10 +856,28 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
             CH.ntiles = v[0]; CH.ntiles32 = v[1]; CH.nsegs = v[2]; CH.nsegs32 = v[3];
             CH.total = v[4] + v[5];
             CH.total32 = v[5];
-            if (CH.d_tiles.alloc(v[0]) || CH.d_tiles32.alloc(v[1]) || CH.d_segs.alloc(v[2]) ||
-                CH.d_segs32.alloc(v[3]) || CH.d_list.alloc(v[4]) || CH.d_list32.alloc(v[5]) ||
-                CH.d_counts.alloc(v[2]) || CH.d_counts32.alloc(v[3]))
-                return -1;
+        }
+        // every class's lists in one allocation
+        Arena& A = c->fock_arena;
+        A.reset();
+        std::vector<int> slot((size_t)kNumQuartetClasses * 8);
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            const long long* v = &sm[(size_t)k * kListCounts];
+            int* sl = &slot[(size_t)k * 8];
+            sl[0] = A.reserve<Tile>(v[0]); sl[1] = A.reserve<Tile>(v[1]);
+            sl[2] = A.reserve<int4>(v[2]); sl[3] = A.reserve<int4>(v[3]);
+            sl[4] = A.reserve<int>(v[4]); sl[5] = A.reserve<int>(v[5]);
+            sl[6] = A.reserve<int>(v[2]); sl[7] = A.reserve<int>(v[3]);
+        }
+        if (A.commit()) return -1;
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            ClassHost& CH = c->classes[k];
+            const long long* v = &sm[(size_t)k * kListCounts];
+            const int* sl = &slot[(size_t)k * 8];
+            A.place(CH.d_tiles, sl[0], v[0]); A.place(CH.d_tiles32, sl[1], v[1]);
+            A.place(CH.d_segs, sl[2], v[2]); A.place(CH.d_segs32, sl[3], v[3]);
+            A.place(CH.d_list, sl[4], v[4]); A.place(CH.d_list32, sl[5], v[5]);
+            A.place(CH.d_counts, sl[6], v[2]); A.place(CH.d_counts32, sl[7], v[3]);
             dst[k] = ListDst{CH.d_tiles.p, CH.d_tiles32.p, CH.d_segs.p, CH.d_segs32.p, nullptr, nullptr,
                              nullptr, nullptr, nullptr, nullptr};
         }
@@ -808,9 +888,18 @@ static int prepare(qk_ctx* c, const qk_opts* o) {
             ClassHost& CH = c->classes[k];
             const long long* v = &rs[(size_t)k * kRunCounts];
             CH.nruns = v[0]; CH.nruns32 = v[1]; CH.nsegruns = v[2]; CH.nsegruns32 = v[3];
-            if (CH.d_runs.alloc(v[0]) || CH.d_runs32.alloc(v[1]) || CH.d_segruns.alloc(v[2]) ||
-                CH.d_segruns32.alloc(v[3]))
-                return -1;
+        }
+        Arena& A = c->fock_run_arena;
+        A.reset();
+        for (int k = 0; k < kNumQuartetClasses; ++k)
+            for (int a = 0; a < kRunCounts; ++a) A.reserve<int2>(rs[(size_t)k * kRunCounts + a]);
+        if (A.commit()) return -1;
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            ClassHost& CH = c->classes[k];
+            const long long* v = &rs[(size_t)k * kRunCounts];
+            const int s0 = k * kRunCounts;
+            A.place(CH.d_runs, s0, v[0]); A.place(CH.d_runs32, s0 + 1, v[1]);
+            A.place(CH.d_segruns, s0 + 2, v[2]); A.place(CH.d_segruns32, s0 + 3, v[3]);
             dst[k].r64 = CH.d_runs.p; dst[k].r32 = CH.d_runs32.p;
             dst[k].sr64 = CH.d_segruns.p; dst[k].sr32 = CH.d_segruns32.p;
         }
@@ -1408,18 +1497,44 @@ static int grad_prepare(qk_ctx* c, const qk_opts* o) {
                 set_err("gradient class %d: more than 2^31 tiles or candidate quartets", k);
                 return -1;
             }
-            const bool light = grad_class_is_light(k);
             c->ngtiles[k] = v[0];
             c->ngtiles32[k] = v[1];
-            if (c->d_gtiles[k].alloc(v[0]) || c->d_gtiles32[k].alloc(v[1])) return -1;
+            c->ngsegs[0][k] = v[2];
+            c->ngsegs[1][k] = v[3];
+        }
+        // every class's lists in one allocation
+        Arena& A = c->grad_arena;
+        A.reset();
+        std::vector<int> slot((size_t)kNumQuartetClasses * 12);
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            const long long* v = &sm[(size_t)k * kListCounts];
+            const bool light = grad_class_is_light(k);
+            int* sl = &slot[(size_t)k * 12];
+            sl[0] = A.reserve<Tile>(v[0]);
+            sl[1] = A.reserve<Tile>(v[1]);
             for (int f = 0; f < 2; ++f) {
-                c->ngsegs[f][k] = v[2 + f];
                 const long long nslot = light ? v[f] : 0;   // a light class's segment owns the slots of its tiles
-                if (c->d_gsegs[f][k].alloc(v[2 + f]) || c->d_gslot0[f][k].alloc(v[2 + f]) ||
-                    c->d_gsteps[f][k].alloc(std::max<long long>(nslot, 1)) ||
-                    c->d_glist[f][k].alloc(std::max<long long>(v[4 + f], 1)) ||
-                    c->d_gcounts[f][k].alloc(std::max<long long>(v[2 + f], 1)))
-                    return -1;
+                sl[2 + 5 * f] = A.reserve<int4>(v[2 + f]);
+                sl[3 + 5 * f] = A.reserve<int>(v[2 + f]);
+                sl[4 + 5 * f] = A.reserve<Tile>(std::max<long long>(nslot, 1));
+                sl[5 + 5 * f] = A.reserve<int>(std::max<long long>(v[4 + f], 1));
+                sl[6 + 5 * f] = A.reserve<int>(std::max<long long>(v[2 + f], 1));
+            }
+        }
+        if (A.commit()) return -1;
+        for (int k = 0; k < kNumQuartetClasses; ++k) {
+            const long long* v = &sm[(size_t)k * kListCounts];
+            const bool light = grad_class_is_light(k);
+            const int* sl = &slot[(size_t)k * 12];
+            A.place(c->d_gtiles[k], sl[0], v[0]);
+            A.place(c->d_gtiles32[k], sl[1], v[1]);
+            for (int f = 0; f < 2; ++f) {
+                const long long nslot = light ? v[f] : 0;
+                A.place(c->d_gsegs[f][k], sl[2 + 5 * f], v[2 + f]);
+                A.place(c->d_gslot0[f][k], sl[3 + 5 * f], v[2 + f]);
+                A.place(c->d_gsteps[f][k], sl[4 + 5 * f], std::max<long long>(nslot, 1));
+                A.place(c->d_glist[f][k], sl[5 + 5 * f], std::max<long long>(v[4 + f], 1));
+                A.place(c->d_gcounts[f][k], sl[6 + 5 * f], std::max<long long>(v[2 + f], 1));
             }
             dst[k] = ListDst{c->d_gtiles[k].p, c->d_gtiles32[k].p, c->d_gsegs[0][k].p, c->d_gsegs[1][k].p,
                              c->d_gslot0[0][k].p, c->d_gslot0[1][k].p, nullptr, nullptr, nullptr, nullptr};
