@@ -172,17 +172,39 @@ __device__ __forceinline__ void stage_bra(const PP& pp, int base, int stride, in
 // Exchange contribution to X[row r of the bra][col]: into the block's LDS rows
 // (xr, n columns per bra function) when the block works on one bra at a time,
 // else straight into X.
-__device__ __forceinline__ void xk_add(unsigned long long* X, unsigned long long* xr, int r, long long grow, int col,
+__device__ __forceinline__ void xk_add(unsigned long long* X, unsigned long long* xr, int r, int grow, int col,
                                        int n, double v) {
     const long long iv = __double2ll_rn(v * 0x1p48);
     if (iv == 0) return;
     if (xr) atomicAdd(xr + (long long)r * n + col, (unsigned long long)iv);
-    else atomicAdd(X + grow * n + col, (unsigned long long)iv);
+    else atomicAdd(X + (long long)grow * n + col, (unsigned long long)iv);
 }
 
 __device__ __forceinline__ void fx_add(unsigned long long* X, long long idx, double v) {
     const long long iv = __double2ll_rn(v * 0x1p48);
     if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
+}
+
+// round(v * 2^48) for |v| < 2^15 in FP32 arithmetic: with a = v 2^16 (exact),
+// its integer part h = trunc(a) and fraction f = a - h (exact: the low bits of
+// a, |f| <= 1 - 2^-24), the result is h 2^32 + rint(f 2^32); as h 2^32 is even
+// and rint is odd-symmetric, this is the integer __double2ll_rn(double(v) *
+// 0x1p48) gives, without FP64 instructions
+__device__ __forceinline__ long long fx_round_f(float v) {
+    const float a = v * 65536.0f;
+    const float h = truncf(a);
+    const unsigned lo = (unsigned)rintf(fabsf(a - h) * 4294967296.0f);
+    const long long hi = (long long)(int)h * 4294967296LL;
+    return a < 0.0f ? hi - (long long)lo : hi + (long long)lo;
+}
+
+// xk_add for an FP32 contribution
+__device__ __forceinline__ void xk_add_f(unsigned long long* X, unsigned long long* xr, int r, int grow, int col,
+                                         int n, float v) {
+    const long long iv = fx_round_f(v);
+    if (iv == 0) return;
+    if (xr) atomicAdd(xr + (long long)r * n + col, (unsigned long long)iv);
+    else atomicAdd(X + (long long)grow * n + col, (unsigned long long)iv);
 }
 
 __device__ __forceinline__ double dn_max(const double* cm, int ns, int A, int B, int C, int D) {
@@ -580,7 +602,7 @@ __device__ __forceinline__ void boys_f32_all(float T, const float* tab, float* F
 }
 
 __device__ __forceinline__ void fx_add_f(unsigned long long* X, long long idx, float v) {
-    const long long iv = __double2ll_rn(double(v) * 0x1p48);
+    const long long iv = fx_round_f(v);
     if (iv != 0) atomicAdd(X + idx, (unsigned long long)iv);
 }
 
@@ -776,35 +798,35 @@ __global__ void __launch_bounds__(256, (OccupancyF<LA, LB, LC, LD>::value)) fock
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, ia, fa + ia, fc + ic, n, double(kw * s));
+                        xk_add_f(Xc, xr, ia, fa + ia, fc + ic, n, kw * s);
                     }
                 for (int ia = 0; ia < NA; ++ia)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ib = 0; ib < NB; ++ib)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, ia, fa + ia, fd + id, n, double(kw * s));
+                        xk_add_f(Xc, xr, ia, fa + ia, fd + id, n, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int ic = 0; ic < NC; ++ic) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, NA + ib, fb + ib, fc + ic, n, double(kw * s));
+                        xk_add_f(Xc, xr, NA + ib, fb + ib, fc + ic, n, kw * s);
                     }
                 for (int ib = 0; ib < NB; ++ib)
                     for (int id = 0; id < ND; ++id) {
                         float s = 0.0f;
                         for (int ia = 0; ia < NA; ++ia)
                             for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, double(kw * s));
+                        xk_add_f(Xc, xr, NA + ib, fb + ib, fd + id, n, kw * s);
                     }
             }
 #undef OUT
         }
 #pragma unroll
         for (int k = 0; k < NA * NB; ++k) {
-            const long long v = wave_sum_i64(__double2ll_rn(double(jab[k]) * 0x1p48));
+            const long long v = wave_sum_i64(fx_round_f(jab[k]));
             if (lane == 0 && v != 0) {
                 if (xr) atomicAdd(xr + (long long)(k / NB) * n + fb + k % NB, (unsigned long long)v);
                 else atomicAdd(a.J + a.jofs[p] + (long long)k * pbs, (unsigned long long)v);
