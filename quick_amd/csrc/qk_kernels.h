@@ -564,8 +564,10 @@ __device__ __forceinline__ void boys_f32(float T, const float* tab, float* F) {
 #pragma unroll
         for (int m = LT - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * (1.0f / float(2 * m + 1));
     } else {
-        F[0] = sqrtf(0.78539816339744831f / T);
-        const float h = 0.5f / T;
+        // T >= kBoysTabTmax: sqrt(pi / 4T) by the hardware reciprocal square root
+        // (no denormal or zero operand here)
+        F[0] = 0.88622692545275801f * __builtin_amdgcn_rsqf(T);
+        const float h = 0.5f * __builtin_amdgcn_rcpf(T);
 #pragma unroll
         for (int m = 1; m <= LT; ++m) F[m] = F[m - 1] * (float(2 * m - 1) * h);
     }
