@@ -299,9 +299,9 @@ def main():
         the timed region), then a few builds with per-kernel HIP events for the
         per-class figures; returns ms/build, G, per-kernel rows, stats"""
         tp = time.perf_counter()
-        eng.prepare(opts)                     # host work lists (once per geometry and option set)
+        eng.prepare(opts)                     # work lists (once per geometry and option set)
         prep_s[label] = time.perf_counter() - tp
-        log(f"{label}: prepare (host work lists) {prep_s[label] * 1e3:.1f} ms")
+        log(f"{label}: prepare (work lists) {prep_s[label] * 1e3:.1f} ms")
         eng.set_timing(False)
         for _ in range(args.warmup):
             eng.fock_device(Dd.data_ptr(), Gd.data_ptr(), opts, sptr)
@@ -451,6 +451,16 @@ def main():
                         "mixed": gpick, "mixed_sweep": mixed,
                         "mixed_vs_fp64_speedup": f64["seconds"] / gpick["seconds"] if gpick else None}
 
+    # the gpu_get2e_-style boundary hands over host arrays: the same build with
+    # the density uploaded and the operator downloaded each time (never `value`)
+    hopts = make_opts(precision=QK_MIXED, fp32_threshold=thr) if precision == "mixed" else make_opts()
+    eng.fock(D, hopts)
+    th = time.perf_counter()
+    for _ in range(3):
+        eng.fock(D, hopts)
+    pcie_ms = (time.perf_counter() - th) / 3 * 1e3
+    log(f"host-array builds (D up, G down): {pcie_ms:.2f} ms/build")
+
     scf = None if args.no_scf else scf_leg(b, eng, syms, torch, dev, stream, sptr)
     b3lyp = None if args.no_water64 else water64_leg(args, torch, dev, stream, sptr)
 
@@ -504,6 +514,8 @@ def main():
             "mixed_vs_fp64_speedup": (ms64 + grads["water32"]["seconds"] * 1e3) /
                                      (ms + grads["water32"]["mixed"]["seconds"] * 1e3)},
         "fp64": {"builds_per_s": 1000.0 / ms64, "ms_per_build": ms64},
+        # host arrays in and out (5 MB up, 5 MB down per build), headline precision
+        "pcie_inclusive_ms_per_build": pcie_ms,
         "mixed_vs_fp64_speedup": ms64 / ms,
         # per geometry: Schwarz bounds on the device, then the host's work lists
         # per option set (outside the timed builds)

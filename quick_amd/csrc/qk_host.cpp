@@ -43,8 +43,10 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
                            hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s);
 void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s);
-void launch_oneint(const OneFuncs& f, const OneSys& sys, double* S, double* H, hipStream_t s);
-void launch_oneint_grad(const OneFuncs& f, const OneSys& sys, const double* D, const double* W,
+void launch_oneint(const OneFuncs& f, const OneSys& sys, const int2* const* pairs, const long long* npairs, double* S,
+                   double* H, hipStream_t s);
+void launch_oneint_grad(const OneFuncs& f, const OneSys& sys, const int2* const* pairs, const long long* npairs,
+                        const double* D, const double* W,
                         unsigned long long* G, hipStream_t s);
 void launch_jpair_scatter(const unsigned long long* J, const int* jofs, const int* pstride, const int* qa,
                           const int* qb, const DevQshells& q, int npair, unsigned long long* X, int n,
@@ -306,6 +308,9 @@ struct qk_ctx {
     // count, shell Qfinal, exponents and coefficients (10 per function), and
     // the nuclear charges (qk_set_charges)
     DBuf<int> d1_atom, d1_lmn, d1_np, d1_qf;
+    // function pairs mu <= nu by total angular momentum l(mu) + l(nu) (0..4)
+    DBuf<int2> d1_pairs[5];
+    long long n1_pairs[5] = {};
     DBuf<double> d1_a, d1_c, d_chg;
     std::vector<double> chg;
     DBuf<unsigned long long> d1_acc;
@@ -573,6 +578,19 @@ extern "C" int qk_set_system(qk_ctx* c, int natom, const double* xyz, int nshell
         if (c->d1_atom.upload(fat) || c->d1_lmn.upload(flmn) || c->d1_np.upload(fnp) || c->d1_qf.upload(fqf) ||
             c->d1_a.upload(fa) || c->d1_c.upload(fc))
             return -1;
+        std::vector<int> fl(nbasis);
+        for (int fn = 0; fn < nbasis; ++fn) fl[fn] = KLMN[3 * fn] + KLMN[3 * fn + 1] + KLMN[3 * fn + 2];
+        std::vector<int2> pl[5];
+        for (int nu = 0; nu < nbasis; ++nu)
+            for (int mu = 0; mu <= nu; ++mu) {
+                const int L = fl[mu] + fl[nu];
+                if (L > 4) { set_err("one-electron integrals: functions above d"); return -1; }
+                pl[L].push_back(make_int2(mu, nu));
+            }
+        for (int L = 0; L < 5; ++L) {
+            if (c->d1_pairs[L].upload(pl[L])) return -1;
+            c->n1_pairs[L] = (long long)pl[L].size();
+        }
         c->chg.clear();
     }
     size_t nn = (size_t)nbasis * nbasis;
@@ -1738,7 +1756,8 @@ static OneFuncs one_funcs(const qk_ctx* c) {
 extern "C" int qk_oneint_device(qk_ctx* c, double* dS, double* dH, void* stream) {
     if (oneint_ready(c, "qk_oneint_device")) return -1;
     const OneSys sys{c->d_xyz.p, c->d_chg.p, c->natom, c->nbasis};
-    launch_oneint(one_funcs(c), sys, dS, dH, (hipStream_t)stream);
+    const int2* pl[5] = {c->d1_pairs[0].p, c->d1_pairs[1].p, c->d1_pairs[2].p, c->d1_pairs[3].p, c->d1_pairs[4].p};
+    launch_oneint(one_funcs(c), sys, pl, c->n1_pairs, dS, dH, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1767,7 +1786,8 @@ extern "C" int qk_oneint_grad_device(qk_ctx* c, const double* dD, const double* 
     if (c->d1_acc.alloc(m + 1)) return -1;
     launch_fill_u64(c->d1_acc.p, m + 1, s);
     const OneSys sys{c->d_xyz.p, c->d_chg.p, c->natom, c->nbasis};
-    launch_oneint_grad(one_funcs(c), sys, dD, dW, c->d1_acc.p, s);
+    const int2* pl[5] = {c->d1_pairs[0].p, c->d1_pairs[1].p, c->d1_pairs[2].p, c->d1_pairs[3].p, c->d1_pairs[4].p};
+    launch_oneint_grad(one_funcs(c), sys, pl, c->n1_pairs, dD, dW, c->d1_acc.p, s);
     launch_grad_finalize(c->d1_acc.p, dgrad, m, c->d1_acc.p + m, s);
     HIPCHK(hipGetLastError());
     return 0;

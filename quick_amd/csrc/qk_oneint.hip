@@ -7,7 +7,11 @@
 // attrashellopt's D dV/dR, shell.f90:1747-1860).
 //
 // One thread per unordered pair of basis functions (mu <= nu), looping over
-// the pair's primitive pairs and the nuclei.  Each integral is built by the
+// the pair's primitive pairs and the nuclei; one kernel per total angular
+// momentum of the pair (the host lists the pairs of each), so that every
+// table below has a compile-time size and compile-time indices and stays in
+// registers (a table indexed at run time would live in per-lane scratch
+// memory).  Each integral is built by the
 // McMurchie-Davidson scheme: per Cartesian direction the Hermite expansion
 // coefficients E^{ij}_t of the Gaussian product (recurrences in i and j from
 // E^{00}_0 = 1), overlap and kinetic from E^{ij}_0, and the attraction of a
@@ -21,10 +25,13 @@
 
 namespace qk {
 
-// QUICK's F_0..F_M (M <= 8) with the order asked for M, as boys_fit does for a
-// compile-time order: the seven-region fit and upward recursion, or near T = 0
-// the Taylor value at M and downward recursion (subs/fmt.f90:12-84)
-__device__ void boys_fit_n(double X, int M, double* F) {
+// QUICK's F_0..F_MMAX with the order M asked for (L <= M <= MMAX): the
+// seven-region fit and upward recursion, or near T = 0 the Taylor value at M
+// and downward recursion (subs/fmt.f90:12-84).  Every loop runs to the
+// compile-time bound with the order as a predicate, so F stays in registers;
+// entries above M are not read.
+template <int MMAX>
+__device__ __forceinline__ void boys_fit_u(double X, int M, double* F) {
     const double E = exp(-X);
     const bool up = (X > 3.0e-5) || (X > 3.0e-7 && M < 4);
     if (up) {
@@ -37,83 +44,118 @@ __device__ void boys_fit_n(double X, int M, double* F) {
         for (int k = 1; k < kBoysFitTerms; ++k) v = fma(v, z, c[k]);
         const double s = 0.88622692545275801365 * RS;
         F[0] = r <= 2 ? fma(v, X + X, E) : fma(v, E, s);
-        for (int m = 1; m <= M; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
+#pragma unroll
+        for (int m = 1; m <= MMAX; ++m) F[m] = (double(2 * m - 1) * F[m - 1] - E) * 0.5 * XINV;
     } else {
         const double twot = X * 2.0;
-        F[M] = (1.0 - X) / double(2 * M + 1);
-        for (int m = M - 1; m >= 0; --m) F[m] = (twot * F[m + 1] + E) / double(m * 2 + 1);
+        double cur = 0.0;
+#pragma unroll
+        for (int m = MMAX; m >= 0; --m) {
+            if (m == M) cur = (1.0 - X) / double(2 * M + 1);
+            else if (m < M) cur = (twot * cur + E) / double(m * 2 + 1);
+            F[m] = cur;
+        }
     }
 }
 
-// E^{ij}_t, t = 0..i+j, of one direction (i, j >= 0; entries beyond i+j are 0)
-__device__ __forceinline__ void herm1d(int i, int j, double PA, double PB, double oo2p, double* E) {
-    double T[8];
+// E^{ij}_t, t = 0..S, of one direction for i + j <= S (entries beyond i + j are
+// 0): the recurrences in i and j from E^{00}_0 = 1, unrolled to S steps with
+// the step count i + j as a predicate
+template <int S>
+__device__ __forceinline__ void herm_u(int i, int j, double PA, double PB, double oo2p, double* E) {
     E[0] = 1.0;
-    for (int t = 1; t < 8; ++t) E[t] = 0.0;
-    int len = 1;
-    for (int s = 0; s < i + j; ++s) {
-        const double X = s < i ? PA : PB;
-        for (int t = 0; t <= len; ++t)
-            T[t] = (t > 0 ? oo2p * E[t - 1] : 0.0) + (t < len ? X * E[t] : 0.0) + (t + 1 < len ? (t + 1) * E[t + 1] : 0.0);
-        ++len;
-        for (int t = 0; t < len; ++t) E[t] = T[t];
+#pragma unroll
+    for (int t = 1; t <= S; ++t) E[t] = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (s < i + j) {
+            const double X = s < i ? PA : PB;
+            double T[S + 1];
+#pragma unroll
+            for (int t = 0; t <= s + 1; ++t)
+                T[t] = (t > 0 ? oo2p * E[t - 1] : 0.0) + (t <= s ? X * E[t] : 0.0) +
+                       (t + 1 <= s ? (t + 1) * E[t + 1] : 0.0);
+#pragma unroll
+            for (int t = 0; t <= s + 1; ++t) E[t] = T[t];
+        }
     }
 }
 
-// 1D overlap factor E^{ij}_0 (0 for a negative power)
-__device__ __forceinline__ double s1d(int i, int j, double PA, double PB, double oo2p) {
+// 1D overlap factor E^{ij}_0 (0 for a negative power), i + j <= S
+template <int S>
+__device__ __forceinline__ double s1d_u(int i, int j, double PA, double PB, double oo2p) {
     if (i < 0 || j < 0) return 0.0;
-    double E[8];
-    herm1d(i, j, PA, PB, oo2p, E);
+    double E[S + 1];
+    herm_u<S>(i, j, PA, PB, oo2p, E);
     return E[0];
 }
 
 // 1D kinetic factor: -1/2 d^2/dx^2 on the second function, in units of the
 // 1D overlap's sqrt(pi/p): b(2j+1) S_ij - 2 b^2 S_i,j+2 - j(j-1)/2 S_i,j-2
-__device__ __forceinline__ double t1d(int i, int j, double b, double PA, double PB, double oo2p) {
+// (i + j + 2 <= S)
+template <int S>
+__device__ __forceinline__ double t1d_u(int i, int j, double b, double PA, double PB, double oo2p) {
     if (i < 0 || j < 0) return 0.0;
-    double t = b * (2 * j + 1) * s1d(i, j, PA, PB, oo2p) - 2.0 * b * b * s1d(i, j + 2, PA, PB, oo2p);
-    if (j >= 2) t -= 0.5 * j * (j - 1) * s1d(i, j - 2, PA, PB, oo2p);
+    double t = b * (2 * j + 1) * s1d_u<S>(i, j, PA, PB, oo2p) - 2.0 * b * b * s1d_u<S>(i, j + 2, PA, PB, oo2p);
+    if (j >= 2) t -= 0.5 * j * (j - 1) * s1d_u<S>(i, j - 2, PA, PB, oo2p);
     return t;
 }
 
-// Hermite Coulomb integrals R^{(0)}_tuv for t <= TX, u <= TY, v <= TZ from F_n:
-// R^{(n)}_000 = (-2p)^n F_n, raised one index at a time; only the entries with
-// n + t + u + v <= L are formed (the contractions read no others).  R is laid
-// out [n][t][u][v] with extents (L+1, TX+1, TY+1, TZ+1): at most 6 x 36 for the
-// gradient of a d-d pair (L = 5, TX + TY + TZ = 7).
-constexpr int kRMax = 6 * 36;
-__device__ void hermite_r(int L, int TX, int TY, int TZ, double p, const double* PC, const double* F, double* R) {
-    const int SY = TZ + 1, SX = (TY + 1) * SY, SN = (TX + 1) * SX;
-    double pw = 1.0;
-    for (int n = 0; n <= L; ++n) {
-        R[n * SN] = pw * F[n];
-        pw *= -2.0 * p;
-    }
-    for (int n = L - 1; n >= 0; --n)
-        for (int t = 0; t <= TX; ++t)
-            for (int u = 0; u <= TY; ++u)
-                for (int v = 0; v <= TZ; ++v) {
-                    if (t + u + v == 0 || n + t + u + v > L) continue;
-                    const double* R1 = R + (n + 1) * SN;
-                    double r;
-                    if (t > 0) r = (t > 1 ? (t - 1) * R1[(t - 2) * SX + u * SY + v] : 0.0) + PC[0] * R1[(t - 1) * SX + u * SY + v];
-                    else if (u > 0) r = (u > 1 ? (u - 1) * R1[t * SX + (u - 2) * SY + v] : 0.0) + PC[1] * R1[t * SX + (u - 1) * SY + v];
-                    else r = (v > 1 ? (v - 1) * R1[t * SX + u * SY + v - 2] : 0.0) + PC[2] * R1[t * SX + u * SY + v - 1];
-                    R[n * SN + t * SX + u * SY + v] = r;
-                }
+// (t, u, v) with t + u + v <= L in one tetrahedral array: the shells of equal
+// t + u + v one after another, within a shell by u + v, then v
+constexpr int tet_size(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+constexpr int tet(int t, int u, int v) {
+    return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 + (u + v) * (u + v + 1) / 2 + v;
 }
 
-// sum_tuv Ex_t Ey_u Ez_v R^{(0)}_tuv
-__device__ __forceinline__ double hermite_contract(const double* Ex, int nx, const double* Ey, int ny, const double* Ez,
-                                                   int nz, const double* R, int TY, int TZ) {
-    const int SY = TZ + 1, SX = (TY + 1) * SY;
+// Hermite Coulomb integrals R^{(0)}_tuv for t + u + v <= L: R^{(n)}_000 =
+// (-2p)^n F_n, raised one index at a time (t first, then u, then v), level n
+// holding t + u + v <= L - n; all indices compile-time, so the table stays in
+// registers
+template <int L>
+__device__ __forceinline__ void hermite_r_u(double p, const double* PC, const double* F, double* R0) {
+    double R[L + 1][tet_size(L)];
+    double pw = 1.0;
+#pragma unroll
+    for (int n = 0; n <= L; ++n) {
+        R[n][0] = pw * F[n];
+        pw *= -2.0 * p;
+    }
+#pragma unroll
+    for (int n = L - 1; n >= 0; --n)
+#pragma unroll
+        for (int s = 1; s <= L - n; ++s)
+#pragma unroll
+            for (int r = 0; r <= s; ++r)
+#pragma unroll
+                for (int v = 0; v <= r; ++v) {
+                    const int u = r - v, t = s - r;
+                    double x;
+                    if (t > 0)
+                        x = (t > 1 ? (t - 1) * R[n + 1][tet(t - 2, u, v)] : 0.0) + PC[0] * R[n + 1][tet(t - 1, u, v)];
+                    else if (u > 0)
+                        x = (u > 1 ? (u - 1) * R[n + 1][tet(t, u - 2, v)] : 0.0) + PC[1] * R[n + 1][tet(t, u - 1, v)];
+                    else
+                        x = (v > 1 ? (v - 1) * R[n + 1][tet(t, u, v - 2)] : 0.0) + PC[2] * R[n + 1][tet(t, u, v - 1)];
+                    R[n][tet(t, u, v)] = x;
+                }
+#pragma unroll
+    for (int k = 0; k < tet_size(L); ++k) R0[k] = R[0][k];
+}
+
+// sum_tuv Ex_t Ey_u Ez_v R^{(0)}_tuv over t + u + v <= L (E zero beyond each
+// direction's i + j, so the sum is that over the box)
+template <int L>
+__device__ __forceinline__ double contract_u(const double* Ex, const double* Ey, const double* Ez, const double* R0) {
     double s = 0.0;
-    for (int t = 0; t <= nx; ++t) {
+#pragma unroll
+    for (int t = 0; t <= L; ++t) {
         double su = 0.0;
-        for (int u = 0; u <= ny; ++u) {
+#pragma unroll
+        for (int u = 0; u <= L - t; ++u) {
             double sv = 0.0;
-            for (int v = 0; v <= nz; ++v) sv += Ez[v] * R[t * SX + u * SY + v];
+#pragma unroll
+            for (int v = 0; v <= L - t - u; ++v) sv += Ez[v] * R0[tet(t, u, v)];
             su += Ey[u] * sv;
         }
         s += Ex[t] * su;
@@ -121,73 +163,72 @@ __device__ __forceinline__ double hermite_contract(const double* Ex, int nx, con
     return s;
 }
 
-// unordered pair k of the triangle (mu <= nu): nu (nu + 1) / 2 + mu = k
-__device__ __forceinline__ void tri_pair(long long k, int& mu, int& nu) {
-    int j = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
-    while ((long long)(j + 1) * (j + 2) / 2 <= k) ++j;
-    while ((long long)j * (j + 1) / 2 > k) --j;
-    nu = j;
-    mu = (int)(k - (long long)j * (j + 1) / 2);
-}
-
 constexpr double kPi = 3.14159265358979323846;
 // primitive pairs whose Gaussian-product prefactor lies below this contribute
 // less than 1e-24 to any element and are skipped
 constexpr double k1eKabCut = 1e-24;
+// Boys orders asked: Qfinal(I) + Qfinal(J) <= 4 (d shells), + 2 for the gradient
+constexpr int kBoys1eMax = 4;
 
-// S and H = T + V (n x n, column-major, both triangles) for every function pair
-__global__ void __launch_bounds__(256) oneint_kernel(const OneFuncs f, const OneSys sys, double* __restrict__ S,
-                                                     double* __restrict__ H) {
-    const long long npair = (long long)sys.n * (sys.n + 1) / 2;
+__device__ __forceinline__ void unpack_lmn(int v, int* l) {
+    l[0] = v & 15;
+    l[1] = (v >> 4) & 15;
+    l[2] = (v >> 8) & 15;
+}
+
+// S and H = T + V (n x n, column-major, both triangles) for the function pairs
+// (mu <= nu) of total angular momentum LT
+template <int LT>
+__global__ void __launch_bounds__(256) oneint_kernel(const OneFuncs f, const OneSys sys, const int2* __restrict__ pairs,
+                                                     long long npair, double* __restrict__ S, double* __restrict__ H) {
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < npair;
          k += (long long)gridDim.x * blockDim.x) {
-    int I, J;
-    tri_pair(k, I, J);
-    const int atA = f.atom[I], atB = f.atom[J];
-    const double A[3] = {sys.xyz[3 * atA], sys.xyz[3 * atA + 1], sys.xyz[3 * atA + 2]};
-    const double B[3] = {sys.xyz[3 * atB], sys.xyz[3 * atB + 1], sys.xyz[3 * atB + 2]};
-    const int li[3] = {f.lmn[I] & 15, (f.lmn[I] >> 4) & 15, (f.lmn[I] >> 8) & 15};
-    const int lj[3] = {f.lmn[J] & 15, (f.lmn[J] >> 4) & 15, (f.lmn[J] >> 8) & 15};
-    const int L = li[0] + li[1] + li[2] + lj[0] + lj[1] + lj[2];
-    const int M = max(f.qf[I] + f.qf[J], L);
-    const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
-    double s = 0.0, t = 0.0, v = 0.0;
-    double R[kRMax], F[9], Ex[8], Ey[8], Ez[8];
-    for (int ci = 0; ci < f.np[I]; ++ci)
-        for (int cj = 0; cj < f.np[J]; ++cj) {
-            const double a = f.a[k1eMaxPrim * I + ci], b = f.a[k1eMaxPrim * J + cj];
-            const double p = a + b, oop = 1.0 / p, oo2p = 0.5 * oop;
-            const double Kab = exp(-a * b * oop * AB2);
-            if (Kab < k1eKabCut) continue;
-            const double cc = f.c[k1eMaxPrim * I + ci] * f.c[k1eMaxPrim * J + cj];
-            double P[3], PA[3], PB[3];
-            for (int d = 0; d < 3; ++d) {
-                P[d] = (a * A[d] + b * B[d]) * oop;
-                PA[d] = P[d] - A[d];
-                PB[d] = P[d] - B[d];
+        const int I = pairs[k].x, J = pairs[k].y;
+        const int atA = f.atom[I], atB = f.atom[J];
+        const double A[3] = {sys.xyz[3 * atA], sys.xyz[3 * atA + 1], sys.xyz[3 * atA + 2]};
+        const double B[3] = {sys.xyz[3 * atB], sys.xyz[3 * atB + 1], sys.xyz[3 * atB + 2]};
+        int li[3], lj[3];
+        unpack_lmn(f.lmn[I], li);
+        unpack_lmn(f.lmn[J], lj);
+        const int M = max(f.qf[I] + f.qf[J], LT);
+        const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
+        double s = 0.0, t = 0.0, v = 0.0;
+        for (int ci = 0; ci < f.np[I]; ++ci)
+            for (int cj = 0; cj < f.np[J]; ++cj) {
+                const double a = f.a[k1eMaxPrim * I + ci], b = f.a[k1eMaxPrim * J + cj];
+                const double p = a + b, oop = 1.0 / p, oo2p = 0.5 * oop;
+                const double Kab = exp(-a * b * oop * AB2);
+                if (Kab < k1eKabCut) continue;
+                const double cc = f.c[k1eMaxPrim * I + ci] * f.c[k1eMaxPrim * J + cj];
+                double P[3], PA[3], PB[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    P[d] = (a * A[d] + b * B[d]) * oop;
+                    PA[d] = P[d] - A[d];
+                    PB[d] = P[d] - B[d];
+                }
+                double E[3][LT + 1];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) herm_u<LT>(li[d], lj[d], PA[d], PB[d], oo2p, E[d]);
+                const double sx = E[0][0], sy = E[1][0], sz = E[2][0];
+                const double tx = t1d_u<LT + 2>(li[0], lj[0], b, PA[0], PB[0], oo2p),
+                             ty = t1d_u<LT + 2>(li[1], lj[1], b, PA[1], PB[1], oo2p),
+                             tz = t1d_u<LT + 2>(li[2], lj[2], b, PA[2], PB[2], oo2p);
+                const double s0 = Kab * kPi * oop * sqrt(kPi * oop);   // (pi/p)^{3/2} K_ab
+                s += cc * s0 * sx * sy * sz;
+                t += cc * s0 * (tx * sy * sz + sx * ty * sz + sx * sy * tz);
+                const double v0 = 2.0 * kPi * oop * Kab;
+                for (int C = 0; C < sys.natom; ++C) {
+                    const double PC[3] = {P[0] - sys.xyz[3 * C], P[1] - sys.xyz[3 * C + 1], P[2] - sys.xyz[3 * C + 2]};
+                    double F[kBoys1eMax + 1], R0[tet_size(LT)];
+                    boys_fit_u<kBoys1eMax>(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), M, F);
+                    hermite_r_u<LT>(p, PC, F, R0);
+                    v -= cc * sys.chg[C] * v0 * contract_u<LT>(E[0], E[1], E[2], R0);
+                }
             }
-            const double sx = s1d(li[0], lj[0], PA[0], PB[0], oo2p), sy = s1d(li[1], lj[1], PA[1], PB[1], oo2p),
-                         sz = s1d(li[2], lj[2], PA[2], PB[2], oo2p);
-            const double tx = t1d(li[0], lj[0], b, PA[0], PB[0], oo2p), ty = t1d(li[1], lj[1], b, PA[1], PB[1], oo2p),
-                         tz = t1d(li[2], lj[2], b, PA[2], PB[2], oo2p);
-            const double s0 = Kab * kPi * oop * sqrt(kPi * oop);   // (pi/p)^{3/2} K_ab
-            s += cc * s0 * sx * sy * sz;
-            t += cc * s0 * (tx * sy * sz + sx * ty * sz + sx * sy * tz);
-            herm1d(li[0], lj[0], PA[0], PB[0], oo2p, Ex);
-            herm1d(li[1], lj[1], PA[1], PB[1], oo2p, Ey);
-            herm1d(li[2], lj[2], PA[2], PB[2], oo2p, Ez);
-            const int TX = li[0] + lj[0], TY = li[1] + lj[1], TZ = li[2] + lj[2];
-            const double v0 = 2.0 * kPi * oop * Kab;
-            for (int C = 0; C < sys.natom; ++C) {
-                const double PC[3] = {P[0] - sys.xyz[3 * C], P[1] - sys.xyz[3 * C + 1], P[2] - sys.xyz[3 * C + 2]};
-                boys_fit_n(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), M, F);
-                hermite_r(L, TX, TY, TZ, p, PC, F, R);
-                v -= cc * sys.chg[C] * v0 * hermite_contract(Ex, TX, Ey, TY, Ez, TZ, R, TY, TZ);
-            }
-        }
-    const long long o1 = (long long)I * sys.n + J, o2 = (long long)J * sys.n + I;
-    if (S) S[o1] = S[o2] = s;
-    if (H) H[o1] = H[o2] = t + v;
+        const long long o1 = (long long)I * sys.n + J, o2 = (long long)J * sys.n + I;
+        if (S) S[o1] = S[o2] = s;
+        if (H) H[o1] = H[o2] = t + v;
     }
 }
 
@@ -202,41 +243,45 @@ struct Prim1e {
     double a, b, oo2p, PA[3], PB[3];
 };
 
+template <int S>
 __device__ __forceinline__ double st_prod(const Prim1e& q, const int* li, const int* lj, bool kin) {
     double sd[3], td[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        sd[d] = s1d(li[d], lj[d], q.PA[d], q.PB[d], q.oo2p);
-        td[d] = kin ? t1d(li[d], lj[d], q.b, q.PA[d], q.PB[d], q.oo2p) : 0.0;
+        sd[d] = s1d_u<S>(li[d], lj[d], q.PA[d], q.PB[d], q.oo2p);
+        td[d] = kin ? t1d_u<S>(li[d], lj[d], q.b, q.PA[d], q.PB[d], q.oo2p) : 0.0;
     }
     return kin ? td[0] * sd[1] * sd[2] + sd[0] * td[1] * sd[2] + sd[0] * sd[1] * td[2] : sd[0] * sd[1] * sd[2];
 }
 
-// grad += sum over function pairs of D dH/dR - W dS/dR (3 x natom, 2^-44 units,
-// accumulated per block in LDS: dynamic LDS of 3 natom int64)
-__global__ void __launch_bounds__(256) oneint_grad_kernel(const OneFuncs f, const OneSys sys, const double* __restrict__ Dm,
+// grad += sum over the function pairs of total angular momentum LT of
+// D dH/dR - W dS/dR (3 x natom, 2^-44 units, accumulated per block in LDS:
+// dynamic LDS of 3 natom int64).  The derivative integrals raise or lower one
+// index, so the Hermite tables run to LG = LT + 1.
+template <int LT>
+__global__ void __launch_bounds__(256) oneint_grad_kernel(const OneFuncs f, const OneSys sys, const int2* __restrict__ pairs,
+                                                          long long npair, const double* __restrict__ Dm,
                                                           const double* __restrict__ Wm,
                                                           unsigned long long* __restrict__ G) {
+    constexpr int LG = LT + 1;
     extern __shared__ unsigned long long s_g[];
     const int m3 = 3 * sys.natom;
     for (int i = threadIdx.x; i < m3; i += blockDim.x) s_g[i] = 0ull;
     __syncthreads();
-    const long long npair = (long long)sys.n * (sys.n + 1) / 2;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < npair;
          k += (long long)gridDim.x * blockDim.x) {
-        int I, J;
-        tri_pair(k, I, J);
+        const int I = pairs[k].x, J = pairs[k].y;
         const int atA = f.atom[I], atB = f.atom[J];
         const double A[3] = {sys.xyz[3 * atA], sys.xyz[3 * atA + 1], sys.xyz[3 * atA + 2]};
         const double B[3] = {sys.xyz[3 * atB], sys.xyz[3 * atB + 1], sys.xyz[3 * atB + 2]};
-        const int li[3] = {f.lmn[I] & 15, (f.lmn[I] >> 4) & 15, (f.lmn[I] >> 8) & 15};
-        const int lj[3] = {f.lmn[J] & 15, (f.lmn[J] >> 4) & 15, (f.lmn[J] >> 8) & 15};
-        const int L = li[0] + li[1] + li[2] + lj[0] + lj[1] + lj[2] + 1;
-        const int M = max(f.qf[I] + f.qf[J] + 2, L);
+        int li[3], lj[3];
+        unpack_lmn(f.lmn[I], li);
+        unpack_lmn(f.lmn[J], lj);
+        const int M = max(f.qf[I] + f.qf[J] + 2, LG);
         const double wt = I == J ? 1.0 : 2.0;   // (I, J) and (J, I)
         const double dIJ = wt * Dm[(long long)I * sys.n + J], wIJ = wt * Wm[(long long)I * sys.n + J];
         const double AB2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]);
         double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0};
-        double R[kRMax], F[10];
         for (int ci = 0; ci < f.np[I]; ++ci)
             for (int cj = 0; cj < f.np[J]; ++cj) {
                 const double a = f.a[k1eMaxPrim * I + ci], b = f.a[k1eMaxPrim * J + cj];
@@ -247,6 +292,7 @@ __global__ void __launch_bounds__(256) oneint_grad_kernel(const OneFuncs f, cons
                 Prim1e q;
                 q.a = a; q.b = b; q.oo2p = 0.5 * oop;
                 double P[3];
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
                     P[d] = (a * A[d] + b * B[d]) * oop;
                     q.PA[d] = P[d] - A[d];
@@ -255,62 +301,69 @@ __global__ void __launch_bounds__(256) oneint_grad_kernel(const OneFuncs f, cons
                 const double s0 = cc * Kab * kPi * oop * sqrt(kPi * oop);
                 // overlap and kinetic: two-centre, d/dB = -d/dA; zero on one atom
                 if (atA != atB)
+#pragma unroll
                     for (int x = 0; x < 3; ++x) {
                         int lp[3] = {li[0], li[1], li[2]}, lm[3] = {li[0], li[1], li[2]};
                         lp[x] += 1;
                         lm[x] -= 1;
-                        const double dS = 2.0 * a * st_prod(q, lp, lj, false) - li[x] * st_prod(q, lm, lj, false);
-                        const double dT = 2.0 * a * st_prod(q, lp, lj, true) - li[x] * st_prod(q, lm, lj, true);
+                        const double dS = 2.0 * a * st_prod<LG + 2>(q, lp, lj, false) -
+                                          li[x] * st_prod<LG + 2>(q, lm, lj, false);
+                        const double dT = 2.0 * a * st_prod<LG + 2>(q, lp, lj, true) -
+                                          li[x] * st_prod<LG + 2>(q, lm, lj, true);
                         const double c = s0 * (dIJ * dT - wIJ * dS);
                         gA[x] += c;
                         gB[x] -= c;
                     }
                 // attraction: one raised/lowered index on A or on B per direction
-                double E0[3][8], Eap[3][8], Eam[3][8], Ebp[3][8], Ebm[3][8];
+                double E0[3][LG + 1], Eap[3][LG + 1], Eam[3][LG + 1], Ebp[3][LG + 1], Ebm[3][LG + 1];
+#pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    herm1d(li[d], lj[d], q.PA[d], q.PB[d], q.oo2p, E0[d]);
-                    herm1d(li[d] + 1, lj[d], q.PA[d], q.PB[d], q.oo2p, Eap[d]);
-                    herm1d(li[d], lj[d] + 1, q.PA[d], q.PB[d], q.oo2p, Ebp[d]);
-                    if (li[d] > 0) herm1d(li[d] - 1, lj[d], q.PA[d], q.PB[d], q.oo2p, Eam[d]);
-                    else for (int t = 0; t < 8; ++t) Eam[d][t] = 0.0;
-                    if (lj[d] > 0) herm1d(li[d], lj[d] - 1, q.PA[d], q.PB[d], q.oo2p, Ebm[d]);
-                    else for (int t = 0; t < 8; ++t) Ebm[d][t] = 0.0;
+                    herm_u<LG>(li[d], lj[d], q.PA[d], q.PB[d], q.oo2p, E0[d]);
+                    herm_u<LG>(li[d] + 1, lj[d], q.PA[d], q.PB[d], q.oo2p, Eap[d]);
+                    herm_u<LG>(li[d], lj[d] + 1, q.PA[d], q.PB[d], q.oo2p, Ebp[d]);
+                    if (li[d] > 0) herm_u<LG>(li[d] - 1, lj[d], q.PA[d], q.PB[d], q.oo2p, Eam[d]);
+                    else
+#pragma unroll
+                        for (int t = 0; t <= LG; ++t) Eam[d][t] = 0.0;
+                    if (lj[d] > 0) herm_u<LG>(li[d], lj[d] - 1, q.PA[d], q.PB[d], q.oo2p, Ebm[d]);
+                    else
+#pragma unroll
+                        for (int t = 0; t <= LG; ++t) Ebm[d][t] = 0.0;
                 }
-                const int TX = li[0] + lj[0] + 1, TY = li[1] + lj[1] + 1, TZ = li[2] + lj[2] + 1;
                 const double v0 = cc * dIJ * 2.0 * kPi * oop * Kab;
                 for (int C = 0; C < sys.natom; ++C) {
                     if (atA == C && atB == C) continue;
                     const double PC[3] = {P[0] - sys.xyz[3 * C], P[1] - sys.xyz[3 * C + 1], P[2] - sys.xyz[3 * C + 2]};
-                    boys_fit_n(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), M, F);
-                    hermite_r(L, TX, TY, TZ, p, PC, F, R);
+                    double F[kBoys1eMax + 3], R0[tet_size(LG)];
+                    boys_fit_u<kBoys1eMax + 2>(p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]), M, F);
+                    hermite_r_u<LG>(p, PC, F, R0);
                     const double z = -sys.chg[C] * v0;
                     double gc[3];
+#pragma unroll
                     for (int x = 0; x < 3; ++x) {
-                        const double* ex[3] = {E0[0], E0[1], E0[2]};
-                        int nx[3] = {li[0] + lj[0], li[1] + lj[1], li[2] + lj[2]};
-                        ex[x] = Eap[x];
-                        nx[x] += 1;
-                        double dA = 2.0 * a * hermite_contract(ex[0], nx[0], ex[1], nx[1], ex[2], nx[2], R, TY, TZ);
+                        const double* e0 = x == 0 ? Eap[0] : E0[0];
+                        const double* e1 = x == 1 ? Eap[1] : E0[1];
+                        const double* e2 = x == 2 ? Eap[2] : E0[2];
+                        double dA = 2.0 * a * contract_u<LG>(e0, e1, e2, R0);
                         if (li[x] > 0) {
-                            ex[x] = Eam[x];
-                            nx[x] -= 2;
-                            dA -= li[x] * hermite_contract(ex[0], nx[0], ex[1], nx[1], ex[2], nx[2], R, TY, TZ);
-                            nx[x] += 2;
+                            dA -= li[x] * contract_u<LG>(x == 0 ? Eam[0] : E0[0], x == 1 ? Eam[1] : E0[1],
+                                                         x == 2 ? Eam[2] : E0[2], R0);
                         }
-                        ex[x] = Ebp[x];
-                        double dB = 2.0 * b * hermite_contract(ex[0], nx[0], ex[1], nx[1], ex[2], nx[2], R, TY, TZ);
+                        double dB = 2.0 * b * contract_u<LG>(x == 0 ? Ebp[0] : E0[0], x == 1 ? Ebp[1] : E0[1],
+                                                             x == 2 ? Ebp[2] : E0[2], R0);
                         if (lj[x] > 0) {
-                            ex[x] = Ebm[x];
-                            nx[x] -= 2;
-                            dB -= lj[x] * hermite_contract(ex[0], nx[0], ex[1], nx[1], ex[2], nx[2], R, TY, TZ);
+                            dB -= lj[x] * contract_u<LG>(x == 0 ? Ebm[0] : E0[0], x == 1 ? Ebm[1] : E0[1],
+                                                         x == 2 ? Ebm[2] : E0[2], R0);
                         }
                         gA[x] += z * dA;
                         gB[x] += z * dB;
                         gc[x] = -z * (dA + dB);
                     }
+#pragma unroll
                     for (int x = 0; x < 3; ++x) lds_gadd(s_g, 3 * C + x, gc[x]);
                 }
             }
+#pragma unroll
         for (int x = 0; x < 3; ++x) {
             lds_gadd(s_g, 3 * atA + x, gA[x]);
             lds_gadd(s_g, 3 * atB + x, gB[x]);
@@ -342,23 +395,43 @@ __global__ void nuclear_grad_kernel(const OneSys sys, unsigned long long* __rest
     }
 }
 
-// grids bounded by the kernels' scratch backing (their per-thread arrays spill);
-// the grid-stride loops cover every pair
-void launch_oneint(const OneFuncs& f, const OneSys& sys, double* S, double* H, hipStream_t s) {
-    const long long npair = (long long)sys.n * (sys.n + 1) / 2;
-    static const long long cap = scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&oneint_kernel)), 4);
-    const long long grid = std::min((npair + 255) / 256, std::min(cap, 256LL * 16));
-    hipLaunchKernelGGL(oneint_kernel, dim3((unsigned)std::max(grid, 1LL)), dim3(256), 0, s, f, sys, S, H);
+// one launch per total angular momentum of the function pairs (lists built
+// on the host once per system: pairs[L] holds the pairs mu <= nu with
+// l(mu) + l(nu) = L); no kernel here spills, so the grids are not bounded by a
+// scratch backing
+template <int LT>
+static void launch_oneint_l(const OneFuncs& f, const OneSys& sys, const int2* pairs, long long np, double* S,
+                            double* H, hipStream_t s) {
+    if (np <= 0) return;
+    const long long grid = std::min((np + 255) / 256, 256LL * 16);
+    hipLaunchKernelGGL(oneint_kernel<LT>, dim3((unsigned)grid), dim3(256), 0, s, f, sys, pairs, np, S, H);
 }
 
-void launch_oneint_grad(const OneFuncs& f, const OneSys& sys, const double* D, const double* W,
-                        unsigned long long* G, hipStream_t s) {
-    const long long npair = (long long)sys.n * (sys.n + 1) / 2;
+void launch_oneint(const OneFuncs& f, const OneSys& sys, const int2* const* pairs, const long long* npairs, double* S,
+                   double* H, hipStream_t s) {
+    launch_oneint_l<0>(f, sys, pairs[0], npairs[0], S, H, s);
+    launch_oneint_l<1>(f, sys, pairs[1], npairs[1], S, H, s);
+    launch_oneint_l<2>(f, sys, pairs[2], npairs[2], S, H, s);
+    launch_oneint_l<3>(f, sys, pairs[3], npairs[3], S, H, s);
+    launch_oneint_l<4>(f, sys, pairs[4], npairs[4], S, H, s);
+}
+
+template <int LT>
+static void launch_oneint_grad_l(const OneFuncs& f, const OneSys& sys, const int2* pairs, long long np,
+                                 const double* D, const double* W, unsigned long long* G, size_t lds, hipStream_t s) {
+    if (np <= 0) return;
+    const long long grid = std::min((np + 255) / 256, 256LL * 16);
+    hipLaunchKernelGGL(oneint_grad_kernel<LT>, dim3((unsigned)grid), dim3(256), lds, s, f, sys, pairs, np, D, W, G);
+}
+
+void launch_oneint_grad(const OneFuncs& f, const OneSys& sys, const int2* const* pairs, const long long* npairs,
+                        const double* D, const double* W, unsigned long long* G, hipStream_t s) {
     const size_t lds = sizeof(unsigned long long) * 3 * (size_t)sys.natom;
-    static const long long cap =
-        scratch_grid_cap(kernel_scratch(reinterpret_cast<const void*>(&oneint_grad_kernel)), 4);
-    const long long grid = std::min((npair + 255) / 256, std::min(cap, 256LL * 16));
-    hipLaunchKernelGGL(oneint_grad_kernel, dim3((unsigned)std::max(grid, 1LL)), dim3(256), lds, s, f, sys, D, W, G);
+    launch_oneint_grad_l<0>(f, sys, pairs[0], npairs[0], D, W, G, lds, s);
+    launch_oneint_grad_l<1>(f, sys, pairs[1], npairs[1], D, W, G, lds, s);
+    launch_oneint_grad_l<2>(f, sys, pairs[2], npairs[2], D, W, G, lds, s);
+    launch_oneint_grad_l<3>(f, sys, pairs[3], npairs[3], D, W, G, lds, s);
+    launch_oneint_grad_l<4>(f, sys, pairs[4], npairs[4], D, W, G, lds, s);
     hipLaunchKernelGGL(nuclear_grad_kernel, dim3((sys.natom + 255) / 256), dim3(256), 0, s, sys, G);
 }
 
