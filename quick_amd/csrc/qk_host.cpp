@@ -996,6 +996,7 @@ static FockArgs make_args(qk_ctx* c, const qk_opts* o, const FockBufs& fb) {
     a.prim_limit = o->prim_limit;
     a.hybrid = o->hybrid;
     a.boys_qshell_order = o->boys_order == QK_BOYS_ORDER_QSHELL ? 1 : 0;
+    a.stage_d = 0;
     return a;
 }
 
@@ -1083,6 +1084,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     // 10.3 -> 7.6 ms), and costs the FP32 classes with an s-s ket (dsss 6.6 ->
     // 9.1 ms) and every d-d bra (on (H2O)32, ddps 1.04 -> 1.77 ms)
     constexpr size_t kRowsLdsMax = 152 * 1024, kRowsLdsSmall = 64 * 1024;
+    constexpr size_t kStageLds = 64 * 1024;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
         if (CH.ntiles == 0 && CH.ntiles32 == 0) continue;
@@ -1096,11 +1098,17 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         // dynamic LDS of the thread-per-quartet kernels: the exchange rows and
         // one bra slab (run mode), or four bra slabs (7 x 64 values each)
         const size_t slab64 = 7 * 64 * sizeof(double), slab32 = 7 * 64 * sizeof(float);
-        const size_t lds64 = use_runs ? rows_lds + slab64 : 4 * slab64;
+        // FP64 run mode: the bra rows of D in LDS too, where they fit kStageLds
+        // (measured on (H2O)32: FP64 build 38.4 -> 37.5 ms, psps 3.14 -> 2.73 ms;
+        // in the FP32 kernels the second code path cost more than it saved)
+        const size_t drows64 = sizeof(double) * (size_t)nrow * n;
+        const bool stage64 = use_runs && rows_lds + slab64 + drows64 <= kStageLds;
+        const size_t lds64 = use_runs ? rows_lds + slab64 + (stage64 ? drows64 : 0) : 4 * slab64;
         const size_t lds32 = use_runs32 ? rows_lds + slab32 : 4 * slab32;
         if (CH.ntiles) {
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[k], s64));
             a.counters = fb->counters + 2 * k;
+            a.stage_d = stage64 ? 1 : 0;
             if (fock_class_is_team(k)) {
                 ClassWork w{CH.d_tiles.p, CH.ntiles, nullptr, 0};
                 // one 64-lane block per tile; up to 16 resident per CU
@@ -1129,6 +1137,7 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         }
         if (mixed && CH.ntiles32) {
             a.counters = fb->counters + 2 * kNumQuartetClasses + 2 * k;
+            a.stage_d = 0;
             const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch_f32(k), 4));
             const int slot = kNumQuartetClasses + k;
             if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));

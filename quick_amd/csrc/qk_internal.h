@@ -304,6 +304,10 @@ struct FockArgs {
     double prim_limit;
     double hybrid;
     int boys_qshell_order;    // 1: Boys order = class L total (reference GPU)
+    // FP64 run mode: the density rows of the run's bra functions ((NA + NB) x n)
+    // staged in LDS after the bra slab, so the exchange contractions read them
+    // there instead of gathering them from memory
+    int stage_d;
 };
 
 }  // namespace qk

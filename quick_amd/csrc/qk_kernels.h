@@ -298,6 +298,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const int wib = threadIdx.x >> 6;
     double* const bra = runs ? reinterpret_cast<double*>(s_xrows + (NA + NB) * n)
                              : reinterpret_cast<double*>(s_xrows) + wib * (7 * 64);
+    double* const sD = runs && a.stage_d ? bra + 7 * 64 : nullptr;   // the run's bra rows of D
     const bool compact = runs && w.segs != nullptr;
     int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
@@ -312,6 +313,13 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
         if (wib == 0) stage_bra(a.pp, a.pr.prim0[rp], a.pr.pstride[rp], a.pr.nprim[rp], lane, bra);
+        if (sD) {
+            const int fa0 = a.q.f0[a.pr.qa[rp]], fb0 = a.q.f0[a.pr.qb[rp]];
+            for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) {
+                const int r = k / n, col = k - r * n;
+                sD[k] = a.Dp[(long long)(r < NA ? fa0 + r : fb0 + r - NA) * n + col];
+            }
+        }
         __syncthreads();
     } else {
         t0 = wave0; t1 = w.ntiles; tstep = nwaves;
@@ -460,34 +468,39 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                     fx_add(a.J, a.jofs[q] + (long long)(ic * ND + id) * pks, jw * s);
                 }
             if (a.hybrid != 0.0) {
-                for (int ia = 0; ia < NA; ++ia)
-                    for (int ic = 0; ic < NC; ++ic) {
-                        double s = 0.0;
-                        for (int ib = 0; ib < NB; ++ib)
-                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fb + ib) * n + fd + id] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, ia, fa + ia, fc + ic, n, (kw * s));
-                    }
-                for (int ia = 0; ia < NA; ++ia)
-                    for (int id = 0; id < ND; ++id) {
-                        double s = 0.0;
-                        for (int ib = 0; ib < NB; ++ib)
-                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fb + ib) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, ia, fa + ia, fd + id, n, (kw * s));
-                    }
-                for (int ib = 0; ib < NB; ++ib)
-                    for (int ic = 0; ic < NC; ++ic) {
-                        double s = 0.0;
-                        for (int ia = 0; ia < NA; ++ia)
-                            for (int id = 0; id < ND; ++id) s += Dp[(long long)(fa + ia) * n + fd + id] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, NA + ib, fb + ib, fc + ic, n, (kw * s));
-                    }
-                for (int ib = 0; ib < NB; ++ib)
-                    for (int id = 0; id < ND; ++id) {
-                        double s = 0.0;
-                        for (int ia = 0; ia < NA; ++ia)
-                            for (int ic = 0; ic < NC; ++ic) s += Dp[(long long)(fa + ia) * n + fc + ic] * OUT(ia, ib, ic, id);
-                        xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, (kw * s));
-                    }
+                // exchange: the bra rows of D from LDS (staged per run) or from memory
+                auto exch = [&](auto DG) {
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int ic = 0; ic < NC; ++ic) {
+                            double s = 0.0;
+                            for (int ib = 0; ib < NB; ++ib)
+                                for (int id = 0; id < ND; ++id) s += DG(NA + ib, fb + ib, fd + id) * OUT(ia, ib, ic, id);
+                            xk_add(Xc, xr, ia, fa + ia, fc + ic, n, (kw * s));
+                        }
+                    for (int ia = 0; ia < NA; ++ia)
+                        for (int id = 0; id < ND; ++id) {
+                            double s = 0.0;
+                            for (int ib = 0; ib < NB; ++ib)
+                                for (int ic = 0; ic < NC; ++ic) s += DG(NA + ib, fb + ib, fc + ic) * OUT(ia, ib, ic, id);
+                            xk_add(Xc, xr, ia, fa + ia, fd + id, n, (kw * s));
+                        }
+                    for (int ib = 0; ib < NB; ++ib)
+                        for (int ic = 0; ic < NC; ++ic) {
+                            double s = 0.0;
+                            for (int ia = 0; ia < NA; ++ia)
+                                for (int id = 0; id < ND; ++id) s += DG(ia, fa + ia, fd + id) * OUT(ia, ib, ic, id);
+                            xk_add(Xc, xr, NA + ib, fb + ib, fc + ic, n, (kw * s));
+                        }
+                    for (int ib = 0; ib < NB; ++ib)
+                        for (int id = 0; id < ND; ++id) {
+                            double s = 0.0;
+                            for (int ia = 0; ia < NA; ++ia)
+                                for (int ic = 0; ic < NC; ++ic) s += DG(ia, fa + ia, fc + ic) * OUT(ia, ib, ic, id);
+                            xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, (kw * s));
+                        }
+                };
+                if (sD) exch([&](int r, int, int col) { return sD[r * n + col]; });
+                else exch([&](int, int grow, int col) { return Dp[(long long)grow * n + col]; });
             }
 #undef OUT
         }
