@@ -257,6 +257,13 @@ def write_units(outdir):
         body = ['// Generated by quick_amd/csrc/gen_eri.py -- do not edit.', '#include "../qk_kernels.h"',
                 'namespace qk {',
                 f'void launch_fock_{t}(const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{',
+            ] + ([
+                '    if (a.stage_d) {',
+                f'        const void* k = reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, true>);',
+                '        if (lds > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);',
+                f'        hipLaunchKernelGGL((fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, true>), dim3(grid), dim3(256), lds, s, a, w);',
+                '        return;',
+                '    }'] if cls not in TEAM_CLASSES else []) + [
                 f'    const void* k = reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>);',
                 '    if (lds > 64 * 1024) (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);',
                 f'    hipLaunchKernelGGL((fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>), dim3(grid), dim3(256), lds, s, a, w);',
@@ -264,7 +271,19 @@ def write_units(outdir):
                 f'size_t fock_scratch_{t}() {{',
                 '    hipFuncAttributes at{};',
                 f'    if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>)) != hipSuccess) return 0;',
-                '    return at.localSizeBytes;',
+            ] + ([
+                '    hipFuncAttributes st{};   // the staged instantiation: the larger of the two',
+                f'    if (hipFuncGetAttributes(&st, reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, true>)) != hipSuccess) return 0;',
+                '    return at.localSizeBytes > st.localSizeBytes ? at.localSizeBytes : st.localSizeBytes;',
+            ] if cls not in TEAM_CLASSES else ['    return at.localSizeBytes;']) + [
+                '}',
+                f'int fock_blocks_{t}(size_t lds, bool staged) {{',
+                '    int nb = 0;',
+                (f'    const void* k = staged ? reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}, true>) : reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>);'
+                 if cls not in TEAM_CLASSES else
+                 f'    const void* k = reinterpret_cast<const void*>(&fock_kernel<{cls[0]}, {cls[1]}, {cls[2]}, {cls[3]}>); (void)staged;'),
+                '    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, lds) != hipSuccess) return 0;',
+                '    return nb;',
                 '}']
         if cls not in TEAM_CLASSES:
             body += [f'void launch_fock_f32_{t}(const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s) {{',

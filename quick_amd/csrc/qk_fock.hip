@@ -199,6 +199,22 @@ void launch_team_f32_class(int k, const FockArgs& a, const ClassWork& w, int gri
     kTeamF32Launchers[k](a, w, grid, s);
 }
 
+#define QK_DECL(T) int fock_blocks_##T(size_t, bool);
+QK_DECL(ssss) QK_DECL(psss) QK_DECL(psps) QK_DECL(ppss) QK_DECL(ppps) QK_DECL(pppp) QK_DECL(dsss)
+QK_DECL(dsps) QK_DECL(dspp) QK_DECL(dsds) QK_DECL(dpss) QK_DECL(dpps) QK_DECL(dppp) QK_DECL(dpds)
+QK_DECL(dpdp) QK_DECL(ddss) QK_DECL(ddps) QK_DECL(ddpp) QK_DECL(ddds) QK_DECL(dddp) QK_DECL(dddd)
+#undef QK_DECL
+using BlocksQuery = int (*)(size_t, bool);
+static const BlocksQuery kFockBlocks[kNumQuartetClasses] = {
+    fock_blocks_ssss, fock_blocks_psss, fock_blocks_psps, fock_blocks_ppss, fock_blocks_ppps, fock_blocks_pppp,
+    fock_blocks_dsss, fock_blocks_dsps, fock_blocks_dspp, fock_blocks_dsds, fock_blocks_dpss, fock_blocks_dpps,
+    fock_blocks_dppp, fock_blocks_dpds, fock_blocks_dpdp, fock_blocks_ddss, fock_blocks_ddps, fock_blocks_ddpp,
+    fock_blocks_ddds, fock_blocks_dddp, fock_blocks_dddd};
+// resident blocks of 256 threads per CU of the FP64 thread-per-quartet kernel
+// of class k (its staged form if staged) with lds bytes of dynamic LDS
+// (registers, LDS and wave limits)
+int fock_class_blocks(int k, size_t lds, bool staged) { return kFockBlocks[k](lds, staged); }
+
 using ScratchQuery = size_t (*)();
 static const ScratchQuery kFockScratch[kNumQuartetClasses] = {
     fock_scratch_ssss, fock_scratch_psss, fock_scratch_psps, fock_scratch_ppss, fock_scratch_ppps, fock_scratch_pppp,

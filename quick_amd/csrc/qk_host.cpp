@@ -26,6 +26,7 @@
 namespace qk {
 void launch_fock_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s);
 size_t fock_class_scratch(int k);
+int fock_class_blocks(int k, size_t lds, bool staged);
 bool fock_class_is_team(int k);
 bool fock_class_has_f32(int k);
 void launch_fock_f32_class(int k, const FockArgs& a, const ClassWork& w, int grid, size_t lds, hipStream_t s);
@@ -301,6 +302,10 @@ struct qk_ctx {
     DBuf<long long> dl_ci, dl_cb, dl_cnt, dl_off, dl_sums, dl_rcnt, dl_roff, dl_rsums;
     DBuf<unsigned char> dl_tmp;
     Arena fock_arena, fock_run_arena, grad_arena;   // the lists themselves
+    // per class, whether the FP64 run-mode kernel stages its bra rows of D in
+    // LDS (-1: not yet decided for stage_n functions)
+    int stage_n = -1;
+    signed char stage_ok[kNumQuartetClasses] = {};
     DBuf<int> d_sh_f0, d_sh_nf;
     DBuf<double> d_Dp, d_cm, d_D, d_G;
     DBuf<unsigned long long> d_X, d_counters;
@@ -1099,10 +1104,20 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
         // one bra slab (run mode), or four bra slabs (7 x 64 values each)
         const size_t slab64 = 7 * 64 * sizeof(double), slab32 = 7 * 64 * sizeof(float);
         // FP64 run mode: the bra rows of D in LDS too, where they fit kStageLds
+        // and leave the kernel as many resident blocks per CU as without them
         // (measured on (H2O)32: FP64 build 38.4 -> 37.5 ms, psps 3.14 -> 2.73 ms;
-        // in the FP32 kernels the second code path cost more than it saved)
+        // on (H2O)64 (ss|ss) staged fell from five blocks per CU to two, 16.6 ->
+        // 21.6 ms; in the FP32 kernels the second code path cost more than it saved)
         const size_t drows64 = sizeof(double) * (size_t)nrow * n;
-        const bool stage64 = use_runs && rows_lds + slab64 + drows64 <= kStageLds;
+        if (c->stage_n != n) {
+            for (int j = 0; j < kNumQuartetClasses; ++j) c->stage_ok[j] = -1;
+            c->stage_n = n;
+        }
+        if (use_runs && c->stage_ok[k] < 0) {
+            const size_t plain = rows_lds + slab64, staged = plain + drows64;
+            c->stage_ok[k] = staged <= kStageLds && fock_class_blocks(k, staged, true) >= fock_class_blocks(k, plain, false) ? 1 : 0;
+        }
+        const bool stage64 = use_runs && c->stage_ok[k] == 1;
         const size_t lds64 = use_runs ? rows_lds + slab64 + (stage64 ? drows64 : 0) : 4 * slab64;
         const size_t lds32 = use_runs32 ? rows_lds + slab32 : 4 * slab32;
         if (CH.ntiles) {

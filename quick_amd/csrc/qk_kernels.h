@@ -270,7 +270,10 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
     return v;
 }
 
-template <int LA, int LB, int LC, int LD>
+// STAGE: run mode only (the host launches it with runs): the run's bra rows of
+// the density staged in LDS after the bra slab, the exchange contractions read
+// there; a separate instantiation, so the unstaged kernel carries no second path
+template <int LA, int LB, int LC, int LD, bool STAGE = false>
 __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_kernel(const FockArgs a, const ClassWork w) {
     constexpr int NE = nrange(LA, LA + LB), NF = nrange(LC, LC + LD);
     constexpr int NACC = NE * NF;
@@ -298,7 +301,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
     const int wib = threadIdx.x >> 6;
     double* const bra = runs ? reinterpret_cast<double*>(s_xrows + (NA + NB) * n)
                              : reinterpret_cast<double*>(s_xrows) + wib * (7 * 64);
-    double* const sD = runs && a.stage_d ? bra + 7 * 64 : nullptr;   // the run's bra rows of D
+    double* const sD = STAGE ? bra + 7 * 64 : nullptr;   // the run's bra rows of D
     const bool compact = runs && w.segs != nullptr;
     int c_seg = 0, c_end = 0, c_step = 0, c_nst = -1, c_gt = 0;   // compacted-run walk
     long long run = blockIdx.x;
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
         c_seg = rg.x; c_end = rg.y; c_step = 0; c_nst = -1; c_gt = 0;
         for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) s_xrows[k] = 0ull;
         if (wib == 0) stage_bra(a.pp, a.pr.prim0[rp], a.pr.pstride[rp], a.pr.nprim[rp], lane, bra);
-        if (sD) {
+        if constexpr (STAGE) {
             const int fa0 = a.q.f0[a.pr.qa[rp]], fb0 = a.q.f0[a.pr.qb[rp]];
             for (int k = threadIdx.x; k < (NA + NB) * n; k += blockDim.x) {
                 const int r = k / n, col = k - r * n;
@@ -499,7 +502,7 @@ __global__ void __launch_bounds__(256, (Occupancy<LA, LB, LC, LD>::value)) fock_
                             xk_add(Xc, xr, NA + ib, fb + ib, fd + id, n, (kw * s));
                         }
                 };
-                if (sD) exch([&](int r, int, int col) { return sD[r * n + col]; });
+                if constexpr (STAGE) exch([&](int r, int, int col) { return sD[r * n + col]; });
                 else exch([&](int, int grow, int col) { return Dp[(long long)grow * n + col]; });
             }
 #undef OUT
