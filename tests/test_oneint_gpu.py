@@ -9,7 +9,10 @@ the reference and the oracle, and the complete HF energy and gradient.
   repulsion, the one-electron terms and the four-centre term, against the
   reference's full hfgrad output (1e-9 Eh/bohr);
 * (H2O)32 (800 functions): H_core samples, the total energy and the complete
-  gradient at the converged density; bitwise repeatability.
+  gradient at the converged density; bitwise repeatability;
+* n-icosane (510 functions) at the density of the reference's own SCF: G and
+  H_core samples, the total energy, the four-centre gradient (FP64 and mixed)
+  and the complete gradient.
 """
 import ctypes as C
 
@@ -102,3 +105,39 @@ def test_oneint_water32_converged():
     assert abs(E - float(z["E_total"])) <= 1e-8
     assert dg <= 1e-9
     assert np.abs(g.sum(axis=1)).max() <= 1e-8           # translational invariance
+
+
+def test_icosane_at_reference_scf_density():
+    """n-icosane C20H42 (510 functions) at the density of the reference's own
+    SCF (tests/golden/icosane_scf.npz: 201 cycles, last RMS density change
+    5e-9): H_core samples, the total energy, the four-centre gradient in FP64
+    and in mixed precision against the reference's grad4c, and the complete
+    gradient from one backward call against its full hfgrad output."""
+    from quick_amd.fock import QK_MIXED
+    _banner("icosane at the reference SCF density")
+    z = np.load("tests/golden/icosane_scf.npz")
+    syms, xyz = CONFIGS["icosane"]()
+    b = build_basis(syms, xyz)
+    b.xyz = np.asarray(z["xyz"], np.float64).copy()
+    eng = FockEngine(b)
+    Y, _ = eng.compute_schwarz()
+    assert np.abs(Y - z["Ycutoff"]).max() <= 1e-12
+    eng.set_charges([ZOF[s] for s in syms])
+    S, H = eng.oneint()
+    dh = np.abs(H[z["H_sample_i"], z["H_sample_j"]] - z["H_sample"]).max()
+    D, W = _dens(z)
+    G = eng.fock(D)
+    dG = np.abs(G[z["G_sample_i"], z["G_sample_j"]] - z["G_sample"]).max()
+    E = float(np.sum(D * H)) + 0.5 * float(np.sum(D * G)) + eng.nuclear_repulsion()
+    g4 = eng.grad(D)
+    g4m = eng.grad(D, make_opts(precision=QK_MIXED, fp32_threshold=1e-2))
+    g = _complete_grad(eng, D, W)
+    d4, d4m, dg = (np.abs(g4 - z["grad4c"]).max(), np.abs(g4m - z["grad4c"]).max(),
+                   np.abs(g - z["grad_total"]).max())
+    print(f"icosane: max|dH| samples {dh:.2e} max|dG| samples {dG:.2e} |dE_total| {abs(E - float(z['E_total'])):.2e} "
+          f"max|dgrad4c| {d4:.2e} (mixed {d4m:.2e}) max|dgrad_total| {dg:.2e}", flush=True)
+    assert dh <= 1e-10 and dG <= 1e-10
+    assert abs(E - float(z["E_total"])) <= 1e-8
+    assert d4 <= 1e-9 and d4m <= 1e-6
+    assert dg <= 1e-9
+    assert np.abs(g.sum(axis=1)).max() <= 1e-8
