@@ -381,9 +381,10 @@ def main():
     t_peak_ms = (total64 / (PEAK_FP64_TFLOPS * 1e12) + total32 / (PEAK_FP32_TFLOPS * 1e12)) * 1e3
 
     # nuclear gradient (four-centre term): the same (H2O)32 system and
-    # n-icosane C20H42 (510 bf), dE2/dR at the fixed synthetic density, in FP64
-    # and in mixed precision, the FP32 threshold autotuned over a sweep: the
-    # fastest whose max |dgrad| against the FP64 gradient is <= 1e-6 Eh/bohr
+    # n-icosane C20H42 (510 bf), dE2/dR at a fixed density (with --density
+    # converged, the density of the reference's own SCF on each), in FP64 and in
+    # mixed precision, the FP32 threshold autotuned over a sweep: the fastest
+    # whose max |dgrad| against the FP64 gradient is <= 1e-6 Eh/bohr
     grads = {}
     gthrs = [float(t) for t in args.grad_thresholds.split(",")]
     for gname in (() if args.no_gradient else ("water32", "icosane")):
@@ -392,7 +393,12 @@ def main():
         else:
             gsyms, gxyz = CONFIGS[gname]()
             gb = build_basis(gsyms, gxyz)
-            gD = synthetic_density(gb.xyz, gb.katom, gb.ksumtype, gb.nbasis, DENSITY_SEED)
+            if args.density == "converged":
+                zi = np.load(ROOT / "tests" / "golden" / "icosane_scf.npz")
+                gb.xyz = np.asarray(zi["xyz"], np.float64).copy()
+                gD = 2.0 * zi["C_occ"] @ zi["C_occ"].T
+            else:
+                gD = synthetic_density(gb.xyz, gb.katom, gb.ksumtype, gb.nbasis, DENSITY_SEED)
             ge = FockEngine(gb)
             ge.compute_schwarz()
         gDd = torch.from_numpy(np.ascontiguousarray(gD.ravel(order="F"))).to(dev)
@@ -492,7 +498,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if precision == "mixed" else "f64",
         "data": ("seeded (H2O)32 geometry with the reference's converged RHF density "
-                 "(tests/golden/water32_scf.npz); n-icosane and (H2O)64: seeded synthetic densities"
+                 "(tests/golden/water32_scf.npz), n-icosane with that of the reference's SCF on it "
+                 "(tests/golden/icosane_scf.npz); (H2O)64: seeded synthetic density"
                  if args.density == "converged" else
                  "synthetic: seeded (H2O)32 geometry, seeded exp(-0.25 R^2) density"),
         "config": {"workload": "(H2O)32 HF/6-31G** two-electron Fock build G(D), 800 bf, tau 1e-11, "
