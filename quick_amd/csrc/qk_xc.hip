@@ -389,7 +389,8 @@ __global__ void __launch_bounds__(256) xc_rho_gemm(const int4* __restrict__ task
     const int* sl = sig + (long long)B.batch * nbasis;
     const double* P0 = ws + B.ws;
     const long long plane = (long long)kXcBatch * B.w;
-    __shared__ double As[16 * kXcTile], Bs[16 * kXcTile];
+    // two LDS slabs: the MFMAs read one while the next is stored into the other
+    __shared__ double As[2][16 * kXcTile], Bs[2][16 * kXcTile];
     __shared__ int s_sj[kXcTile];
     __shared__ double s_red[2][kXcTile][4];
     const int t = threadIdx.x;
@@ -398,20 +399,40 @@ __global__ void __launch_bounds__(256) xc_rho_gemm(const int4* __restrict__ task
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
     __syncthreads();
-    for (int k0 = 0; k0 < B.w; k0 += 16) {
+    // the slab at k0 into registers: A = Phi rows (points) x k, k contiguous in
+    // memory; B = D[sig k][sig j]
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int e = t + 256 * q;
-            // A = Phi rows (points) x k: k contiguous in memory
             const int ia = e >> 4, ka = e & 15;
-            As[ka * kXcTile + ia] = P0[(long long)(pr0 + ia) * B.w + k0 + ka];
-            // B = D[sig k][sig j]
+            ra[q] = P0[(long long)(pr0 + ia) * B.w + k0 + ka];
             const int kb = e >> 6, jb = e & 63;
             const int gk = k0 + kb < B.m ? sl[k0 + kb] : -1, gj = s_sj[jb];
-            Bs[kb * kXcTile + jb] = (gk >= 0 && gj >= 0) ? D[(long long)gk * nbasis + gj] : 0.0;
+            rb[q] = (gk >= 0 && gj >= 0) ? D[(long long)gk * nbasis + gj] : 0.0;
         }
-        __syncthreads();
-        mfma_slab(acc, As, Bs);
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            As[buf][(e & 15) * kXcTile + (e >> 4)] = ra[q];
+            Bs[buf][(e >> 6) * kXcTile + (e & 63)] = rb[q];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    // slab it is in buffer it & 1; the loads of slab it + 1 are in flight
+    // during its MFMAs, and the barrier at the end of step it orders both the
+    // reads of buffer it & 1 before its next stores and the stores of slab
+    // it + 1 before its reads
+    for (int k0 = 0, it = 0; k0 < B.w; k0 += 16, ++it) {
+        const bool more = k0 + 16 < B.w;
+        if (more) load(k0 + 16);
+        mfma_slab(acc, As[it & 1], Bs[it & 1]);
+        if (more) store((it & 1) ^ 1);
         __syncthreads();
     }
     // epilogue: row dots with Phi, dPhi/dx, dPhi/dy, dPhi/dz over the tile's columns
@@ -512,27 +533,46 @@ __global__ void __launch_bounds__(256) xc_v_gemm(const int4* __restrict__ tasks,
     const int* sl = sig + (long long)B.batch * nbasis;
     const double* P0 = ws + B.ws;
     const double* Z = P0 + 4 * (long long)kXcBatch * B.w;
-    __shared__ double As[16 * kXcTile], Bs[16 * kXcTile];
+    __shared__ double As[2][16 * kXcTile], Bs[2][16 * kXcTile];
     const int t = threadIdx.x;
     v4d acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int pass = 0; pass < 2; ++pass) {
-        const double* L = pass == 0 ? P0 : Z;   // A(i, k) = L[k][I0 + i]
-        const double* R = pass == 0 ? Z : P0;   // B(k, j) = R[k][J0 + j]
-        for (int k0 = 0; k0 < kXcBatch; k0 += 16) {
+    // step st: pass st / nk (0: A(i, k) = Phi[k][I0 + i], B(k, j) = Z[k][J0 + j];
+    // 1: the operands swapped), points 16 (st % nk) ..; loaded into registers
+    // one step ahead and stored into the LDS slab the MFMAs are not reading,
+    // as in xc_rho_gemm
+    constexpr int nk = kXcBatch / 16;
+    double ra[4], rb[4];
+    auto load = [&](int st) {
+        const double* L = st < nk ? P0 : Z;
+        const double* R = st < nk ? Z : P0;
+        const int k0 = 16 * (st % nk);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int e = t + 256 * q;
-                const int kk = e >> 6, ii = e & 63;
-                const long long row = (long long)(k0 + kk) * B.w;
-                As[kk * kXcTile + ii] = L[row + I0 + ii];
-                Bs[kk * kXcTile + ii] = R[row + J0 + ii];
-            }
-            __syncthreads();
-            mfma_slab(acc, As, Bs);
-            __syncthreads();
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const long long row = (long long)(k0 + (e >> 6)) * B.w;
+            ra[q] = L[row + I0 + (e & 63)];
+            rb[q] = R[row + J0 + (e & 63)];
         }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            As[buf][e] = ra[q];   // (e >> 6) * kXcTile + (e & 63)
+            Bs[buf][e] = rb[q];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int st = 0; st < 2 * nk; ++st) {
+        const bool more = st + 1 < 2 * nk;
+        if (more) load(st + 1);
+        mfma_slab(acc, As[st & 1], Bs[st & 1]);
+        if (more) store((st & 1) ^ 1);
+        __syncthreads();
     }
     const int lane = t & 63, v = t >> 6;
     const int r0 = 32 * (v >> 1), c0 = 32 * (v & 1);
