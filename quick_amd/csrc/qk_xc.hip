@@ -512,10 +512,18 @@ __global__ void __launch_bounds__(256) xc_point(const XcBatch* __restrict__ bt, 
     __syncthreads();
     double* P0 = ws + B.ws;
     const long long plane = (long long)kXcBatch * B.w;
-    const long long tot = plane;
-    for (long long e = threadIdx.x; e < tot; e += 256) {
-        const int pp = int(e / B.w);
+    // element e = pp w + k: point pp, column k, advanced by 256 elements per
+    // step without a division (k + dr < 2 w)
+    const int w = B.w, dq = 256 / w, dr = 256 % w;
+    int pp = threadIdx.x / w, k = threadIdx.x % w;
+    for (long long e = threadIdx.x; e < plane; e += 256) {
         const double* c = s_c[pp];
+        pp += dq;
+        k += dr;
+        if (k >= w) {
+            k -= w;
+            ++pp;
+        }
         P0[4 * plane + e] = c[0] * P0[e] + c[1] * P0[plane + e] + c[2] * P0[2 * plane + e] + c[3] * P0[3 * plane + e];
     }
 }
