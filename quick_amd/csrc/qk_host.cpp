@@ -1090,6 +1090,17 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
     // 9.1 ms) and every d-d bra (on (H2O)32, ddps 1.04 -> 1.77 ms)
     constexpr size_t kRowsLdsMax = 152 * 1024, kRowsLdsSmall = 64 * 1024;
     constexpr size_t kStageLds = 64 * 1024;
+    // With light FP64 lists (under a tenth of the tiles) the second stream
+    // ends first, so the FP32 lists of (dd|dp) and (dd|dd) follow the FP64
+    // lists there.  Measured, mixed: (H2O)32 at 1e-3 (FP64 lists 7.5% of the
+    // tiles) 23.48 -> 22.98 ms, (H2O)64 at exchange weight 0.2 and 1e-3 (5.5%)
+    // 145.4 -> 144.0 ms; at 1e-4 (13.1%) the same move cost 23.86 -> 24.33 ms.
+    long long tiles64 = 0, tiles32 = 0;
+    for (int k = 0; k < kNumQuartetClasses; ++k) {
+        tiles64 += c->classes[k].ntiles;
+        tiles32 += c->classes[k].ntiles32;
+    }
+    const bool light64 = mixed && tiles64 * 10 < tiles64 + tiles32;
     for (int k : order) {
         ClassHost& CH = c->classes[k];
         if (CH.ntiles == 0 && CH.ntiles32 == 0) continue;
@@ -1151,18 +1162,19 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
             c->cls_timed[k] = c->timing;
         }
         if (mixed && CH.ntiles32) {
+            const hipStream_t ls = light64 && (k == 19 || k == 20) ? s64 : s;
             a.counters = fb->counters + 2 * kNumQuartetClasses + 2 * k;
             a.stage_d = 0;
             const long long cap = std::min(256LL * 16, scratch_grid_cap((long long)fock_class_scratch_f32(k), 4));
             const int slot = kNumQuartetClasses + k;
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t0[slot], ls));
             ClassWork w{CH.d_tiles32.p, CH.ntiles32, nullptr, 0};
             int grid = (int)std::min<long long>((CH.ntiles32 + 3) / 4, cap);
             size_t lds = lds32;
             // one 64-lane block per tile; up to 20 resident per CU (five wavefronts per SIMD)
             if (fock_class_is_team(k)) grid = (int)std::min<long long>(CH.ntiles32, 256LL * 20);
             if (use_runs32 && CH.compact32 && fb->compact) {
-                launch_compact(ca, CH.d_segs32.p, CH.nsegs32, CH.d_list32.p, CH.d_counts32.p, s);
+                launch_compact(ca, CH.d_segs32.p, CH.nsegs32, CH.d_list32.p, CH.d_counts32.p, ls);
                 w.runs = CH.d_segruns32.p;
                 w.nruns = CH.nsegruns32;
                 w.segs = CH.d_segs32.p;
@@ -1176,10 +1188,10 @@ static int fock_enqueue(qk_ctx* c, const double* dD, const double* dcutmat_or_nu
                 grid = (int)std::min<long long>(w.nruns, cap);
                 lds = lds32;
             }
-            if (fock_class_is_team(k)) launch_team_f32_class(k, a, w, grid, s);
-            else launch_fock_f32_class(k, a, w, grid, lds, s);
+            if (fock_class_is_team(k)) launch_team_f32_class(k, a, w, grid, ls);
+            else launch_fock_f32_class(k, a, w, grid, lds, ls);
             HIPCHK(hipGetLastError());
-            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], s));
+            if (c->timing) HIPCHK(hipEventRecord(c->ev_t1[slot], ls));
             c->cls_timed[slot] = c->timing;
         }
     }
