@@ -136,7 +136,9 @@ int qk_prepare(qk_ctx* c, const qk_opts* o);
 
 /* Forward: dG <- G(dD), the two-electron operator (full symmetric n x n) for
  * the density dD; both device pointers, n x n doubles.  Runs on `stream`
- * (a hipStream_t, NULL = default) and returns after enqueueing. */
+ * (a hipStream_t, NULL = default) and returns after enqueueing; part of the
+ * work runs on a second stream of the context, forked from `stream` and joined
+ * back to it with events. */
 int qk_fock_device(qk_ctx* c, const double* dD, double* dG, const qk_opts* o, void* stream);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory during the call (code under HIP graph capture, tight serving loops,
@@ -162,7 +164,8 @@ int qk_fock_host(qk_ctx* c, const double* D, double* F, const qk_opts* o);
  * shell.f90:1863-2535, hrr.f90:862-1014).  opts->tau is the gradient cutoff
  * (quick_method%gradCutoff).  With opts->precision QK_MIXED, quartets whose
  * Schwarz bound Y_ij*Y_kl lies below opts->fp32_threshold are differentiated
- * in FP32 (accumulated in FP64 fixed point).  Runs on `stream`. */
+ * in FP32 (accumulated in FP64 fixed point).  Runs on `stream`, with a second
+ * stream of the context forked from it and joined back, as qk_fock_device. */
 int qk_grad_device(qk_ctx* c, const double* dD, double* dgrad, const qk_opts* o, void* stream);
 int qk_grad_host(qk_ctx* c, const double* D, double* grad, const qk_opts* o);
 

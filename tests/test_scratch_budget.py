@@ -3,10 +3,8 @@
 The runtime backs a dispatch's scratch with per-lane scratch x 64 lanes x the
 waves the grid can have in flight.  The library bounds the grid of every
 spilling kernel (qk_scratch_grid_cap) so that this stays under 256 MiB per
-dispatch; the classes of one list run one after another on one stream, and a
-mixed-precision build or gradient runs its FP64 lists (a gradient also the
-FP32 lists of a few d classes, after them) on a second stream, so at most two
-backings are live at a time.  Here the per-lane scratch is read
+dispatch; every build and gradient runs its class kernels on two streams, one
+after another on each, so at most two backings are live at a time.  Here the per-lane scratch is read
 from the built gfx950 code objects (no device needed) and the bound is checked
 for every kernel.
 """
