@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kGradTeam, kMinWaves) grad_kernel(const GradAr
     const int nSlots = tab[kGhSlots], nLevels = tab[kGhLevels], LT2 = tab[kGhLT] + 2;
     const int n0 = tab[kGhN0], nA = tab[kGhNA], nB = tab[kGhNB], nC = tab[kGhNC];
     const int nS = n0 + nA + nB + nC;
-    const int NOUT = tab[kGhNOut], CA = tab[kGhCA], CB = tab[kGhCB], CC = tab[kGhCC], CD = tab[kGhCD];
+    const int NOUT = tab[kGhNOut], CB = tab[kGhCB], CC = tab[kGhCC], CD = tab[kGhCD];
     const int nStages = tab[kGhStages];
     const int RV = nSlots > tab[kGhMaxStage] ? nSlots : tab[kGhMaxStage];
     const uint4* nodes = reinterpret_cast<const uint4*>(tab + tab[kGhOffNodes]);
