@@ -369,7 +369,9 @@ def main():
     # the roofline kernel: the longest launch of the build's critical path.  In a
     # mixed build the FP64 lists (the strongest pairs) run on a second stream
     # beside the FP32 lists, so their event times overlap other kernels; the
-    # FP32 lists' stream is the critical path
+    # FP32 lists' stream is the critical path.  In an FP64 build the classes
+    # alternate between two streams, so every launch's event time includes the
+    # overlap with the other stream's kernel
     crit = [r for r in rows if r[6] == "fp32"] if precision == "mixed" else rows
     dom = max(crit, key=lambda r: r[5])
     kname, tag, nq, npq, flops, kms, kprec = dom
