@@ -217,3 +217,10 @@ def test_grad_mixed_full_size(system):
     g0 = eng.grad(D, make_opts(precision=QK_MIXED, fp32_threshold=0.0))
     assert _fp32_share(eng)[0] == 0.0
     assert np.array_equal(g0, g64)
+    # both streams in use (FP64: alternating classes; mixed at 1e-2 on (H2O)32:
+    # light FP64 lists with FP32 lists after them): repeated calls bit for bit
+    assert np.array_equal(eng.grad(D), g64)
+    o2 = make_opts(precision=QK_MIXED, fp32_threshold=1e-2)
+    g2 = eng.grad(D, o2)
+    assert np.array_equal(eng.grad(D, o2), g2)
+    assert np.abs(g2 - g64).max() <= TOL_G_MIXED
