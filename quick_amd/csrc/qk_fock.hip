@@ -39,7 +39,9 @@ namespace qk {
 // segment's tiles (same order as the tiles; the gradient's light kernels
 // stride over them).
 __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const int4* __restrict__ segs,
-                                                      long long nsegs, int* __restrict__ list, int* __restrict__ counts) {
+                                                      long long nsegs, int* __restrict__ list, int* __restrict__ counts,
+                                                      int stage_rows) {
+    extern __shared__ double s_cmrows[];
     const long long s = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (s >= nsegs) return;   // the whole wavefront
@@ -47,6 +49,22 @@ __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const
     const int shp = a.psh[sg.x], atp = a.pat[sg.x];
     const int sA = shp & 0xffff, sB = shp >> 16;
     const int atA = atp & 0xffff, atB = atp >> 16;
+    // with stage_rows, the segment's bra shell rows A and B of the density
+    // maxima in LDS (2 nshell values per wavefront): the four cross terms of
+    // each candidate's test read them there instead of gathering from memory
+    const int ns = a.nshell;
+    double* rA = s_cmrows + (size_t)(threadIdx.x >> 6) * 2 * ns;
+    double* rB = rA + ns;
+    if (stage_rows) {
+        for (int j = lane; j < ns; j += 64) {
+            rA[j] = a.cutmat[(long long)sA * ns + j];
+            rB[j] = a.cutmat[(long long)sB * ns + j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const double m_ab = 4.0 * a.cutmat[(long long)sA * ns + sB];
     const double yp = a.Y[sg.x];
     int cnt = 0;
     for (int q0 = sg.y; q0 < sg.z; q0 += 64) {
@@ -54,7 +72,13 @@ __global__ void __launch_bounds__(256) compact_kernel(const CompactArgs a, const
         bool keep = false;
         if (q < sg.z) {
             const int shq = a.psh[q];
-            const double dn = dn_max(a.cutmat, a.nshell, sA, sB, shq & 0xffff, shq >> 16);
+            const int sC = shq & 0xffff, sD = shq >> 16;
+            // dn_max's value (fmax picks one of its operands, so the grouping
+            // does not change it)
+            const double dn = stage_rows
+                                  ? fmax(fmax(m_ab, 4.0 * a.cutmat[(long long)sC * ns + sD]),
+                                         fmax(fmax(rA[sD], rA[sC]), fmax(rB[sC], rB[sD])))
+                                  : dn_max(a.cutmat, ns, sA, sB, sC, sD);
             if (a.grad) {
                 const int atq = a.pat[q];
                 const int atC = atq & 0xffff, atD = atq >> 16;
@@ -266,7 +290,12 @@ void launch_density_cutoff(const double* D, const int* sh_f0, const int* sh_nf, 
 }
 void launch_compact(const CompactArgs& a, const int4* segs, long long nsegs, int* list, int* counts, hipStream_t s) {
     if (nsegs <= 0) return;
-    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((nsegs + 3) / 4)), dim3(256), 0, s, a, segs, nsegs, list, counts);
+    // rows staged while the block's four wavefronts need at most 64 KB
+    // (nshell <= 1024)
+    const size_t rows = sizeof(double) * 2 * 4 * (size_t)a.nshell;
+    const int stage = rows <= 64 * 1024 ? 1 : 0;
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((nsegs + 3) / 4)), dim3(256), stage ? rows : 0, s, a, segs,
+                       nsegs, list, counts, stage);
 }
 void launch_fill_u64(unsigned long long* p, long long count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(fill_u64, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
